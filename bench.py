@@ -1,0 +1,147 @@
+"""Benchmark of the RGF energy loop on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+
+Workload: the disordered-wire transmission configuration N_pw = 317, N_z = 1000
+(Lx = Ly = 40 nm, dz = 0.25 nm, Gaussian-correlated disorder sigma = 3 meV, xi = 5 nm,
+seeded), energies from the uniform 0-30 meV grid of 4096 points.  A step is one
+batched launch of the transmission sweep over B energies (default: one job per
+resident workgroup slot, 256).  Inputs (V-hat table, basis) are resident in HBM before
+the timed region.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RGF energy-points/s at N_pw=317,N_z=1000 (mixed vs FP64 GPU, max|ΔT|≤1e-6)"
+PEAK_FP64_TFLOPS = 78.6
+
+
+def flops_per_point(N, Nz):
+    # 8 N^3 per complex inverse (zgetrf + zgetri) and per complex N^3 product:
+    # N_z inverses and N_z - 1 products per transmission energy point.
+    return 8.0 * N ** 3 * (2 * Nz - 1)
+
+
+def _cpu_worker(args):
+    # runs in a fresh (spawned) process with OPENBLAS_NUM_THREADS=1; CPU only
+    import numpy as np
+    from oracle import negf_oracle as O
+    vrec, NGX, NGY, atomic_kw, energy = args
+    a = O.Atomic(**atomic_kw)
+    nx, ny = O.planewave_basis(a.ENCUT, a.LX, a.LY)
+    s = O.System(a, nx, ny, vrec, NGX, NGY)
+    t0 = time.perf_counter()
+    tau = O.transmission_tau(s, energy, 0, faithful_solver=False)
+    return time.perf_counter() - t0, tau
+
+
+def cpu_baseline(vrec, NGX, NGY, atomic, energies, workers):
+    """The oracle (numpy + scipy's OpenBLAS zgetrf/zgetri/zgemm, the routines the
+    reference links) timed on the host: `workers` single-threaded processes, one energy
+    each, round-robin like the reference's MPI ranks; rate = energies / wall."""
+    import multiprocessing as mp
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    os.environ["OMP_NUM_THREADS"] = "1"
+    import dataclasses
+    akw = dataclasses.asdict(atomic)
+    jobs = [(vrec, NGX, NGY, akw, float(e)) for e in energies[:workers]]
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(workers) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    wall = time.perf_counter() - t0
+    return len(jobs) / wall, wall, [r[1] for r in res]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-workers", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the RGF jobs are independent and would shard by energy, see DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from negftrans_amd import negf, workloads
+
+    wl = workloads.disorder_c1()
+    NGX, NGY = wl.ngx()
+    atomic = wl.atomic()
+    nx, ny = wl.basis()
+    vrec = negf.local_potential_real_to_planewave(wl.V_real_eV / negf.HARTREE, NGX, NGY)
+    dev = negf.NegfDevice(nx, ny, atomic, vrec, NGX, NGY, flags=negf.PREC_FP64)
+    N, Nz = dev.N, dev.N_z
+    Eall = wl.energies_hartree()
+    batch = args.batch or 256
+    print(f"# workload {wl.name}: N={N} N_z={Nz} batch={batch} steps={args.steps} "
+          f"warmup={args.warmup}", file=sys.stderr, flush=True)
+
+    def energies_for(step):
+        idx = (np.arange(batch) * 16 + step) % len(Eall)   # strided sweep of the 4096 grid
+        return Eall[idx]
+
+    for w in range(args.warmup):
+        dev.transmission(energies_for(w))
+        print(f"# warmup {w} done", file=sys.stderr, flush=True)
+    kernel_s = []
+    results = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        results.append(dev.transmission(energies_for(args.warmup + k)))
+        kernel_s.append(dev.last_kernel_seconds)
+        print(f"# step {k}: kernel {kernel_s[-1]:.3f} s", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    points = batch * args.steps
+    value = points / wall
+    fl = flops_per_point(N, Nz)
+    avg_launch = float(np.mean(kernel_s))
+    achieved = fl * batch / avg_launch / 1e12
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "energy-points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Gaussian-correlated disorder)",
+        "config": {"workload": "disorder_c1 transmission", "N_pw": N, "N_z": Nz,
+                   "energies_per_step": batch, "precision": "fp64"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "k_transmission", "flop_per_point": fl,
+                     "avg_launch_s": avg_launch},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        workers = args.cpu_workers or min(16, os.cpu_count() or 1)
+        sample = Eall[1000:1000 + workers]
+        rate, cwall, taus = cpu_baseline(np.asarray(vrec), NGX, NGY, atomic, sample, workers)
+        line["cpu_baseline"] = {"value": rate, "unit": "energy-points/s", "cores": workers,
+                                "kind": "port",
+                                "sample": f"{workers} energies of the same workload, one per "
+                                          f"single-threaded worker, wall {cwall:.1f} s"}
+        gpu_tau = dev.transmission(sample)
+        line["cpu_baseline"]["max_abs_dT_vs_gpu"] = float(np.max(np.abs(np.array(taus) - gpu_tau)))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

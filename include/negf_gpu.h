@@ -1,0 +1,163 @@
+/*
+ * negf_gpu.h -- C-ABI of the MI355X RGF energy-loop library (libnegf_gpu.so).
+ *
+ * Plain C types only: int32 integers, double, and complex*16 passed as
+ * interleaved (re, im) double pairs (`double _Complex` / Fortran complex(c_double_complex)
+ * are layout-compatible).  All arrays are Fortran column-major, 1-based in Fortran
+ * and 0-based here.  Every call is synchronous with respect to the host: results
+ * are in caller memory when it returns.  One host thread per context at a time.
+ *
+ * Return status (all entry points): 0 = OK, < 0 = bad argument or HIP error
+ * (negf_last_error() gives the message), > 0 = 1-based slice index at which an
+ * exactly-zero pivot was met (the reference ignores LAPACK's INFO and lets the
+ * resulting Inf/NaN propagate; the library does the same and only reports it).
+ *
+ * Citations are to the NEGFtrans Fortran sources (src/...:line).
+ */
+#ifndef NEGF_GPU_H
+#define NEGF_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct negf_ctx negf_ctx;
+
+/* Precision of the bulk block products (flags of negf_ctx_create). */
+#define NEGF_PREC_FP64       0   /* complex128 everywhere (parity path)              */
+#define NEGF_PREC_MIXED_FP32 1   /* FP32 bulk, FP64 pivots/diagonal, FP64 refinement */
+/* V-hat window: faithful copies only |p|<=NGX/2, |q|<=NGY/2 (src/main.f90:242-246). */
+#define NEGF_FULL_WINDOW     16  /* extension: use the whole +-NGX, +-NGY window    */
+
+/* Library version string and last error message (thread-local). */
+const char *negf_version(void);
+const char *negf_last_error(void);
+/* Number of visible GPUs (0 when none); does not create a context. */
+int negf_device_count(void);
+
+/*
+ * Create a context for one device structure (one transport system).
+ *   N        : plane-wave basis size (size(nx_grid)), N >= 1
+ *   N_z      : number of z slices, N_z >= 2
+ *   nx_grid, ny_grid : basis (src/grid.f90:410-431), length N
+ *   Lx, Ly, Lz, V_L, V_R, eta : atomic units as in t_parameters (src/global.f90:5-7)
+ *   vrec     : V_reciprocal_all(-NGX:NGX, -NGY:NGY, 1:N_z) complex128 (src/main.f90:218),
+ *              uploaded once and kept resident on the GPU
+ * Replaces the per-job setup that src/negf.f90:139-150 repeats for every energy.
+ */
+int negf_ctx_create(negf_ctx **out, int N, int N_z, const int *nx_grid, const int *ny_grid,
+                    double Lx, double Ly, double Lz, double V_L, double V_R, double eta,
+                    const double *vrec, int NGX, int NGY, int flags);
+int negf_ctx_destroy(negf_ctx *ctx);
+
+/*
+ * Batched transmission (replaces the job loop src/main.f90:384-398 over
+ * Transmission_Coefficient, src/negf.f90:115-174):
+ *   tau[ie] += sum_ik kw[ik] * tau(E[ie], k = (kx[ik], ky[ik]))      (ie < nE, ik < nk)
+ * E in Hartree (real; +i*eta is added as at src/negf.f90:149).  All nE*nk jobs run
+ * in one persistent launch.  tau_jobs (optional, may be NULL) receives the per-job
+ * tau before the k weight, ordered job = ie + nE*ik (the reference's i_job - 1).
+ */
+int negf_transmission_batch(negf_ctx *ctx, const double *E, int nE, const double *kx,
+                            const double *ky, const double *kw, int nk, double *tau,
+                            double *tau_jobs);
+
+/*
+ * Drop-in for GreensFunction_tri_solver(G_inv_D, G_Function, DFL)
+ * (src/math_kernel.f90:44-165): off-diagonal blocks are identities.
+ *   G_inv_D : N*N*N_z complex128
+ *   mode    : bit mask, 1 = "D" (diagonal), 2 = "F" (first column), 4 = "L" (last column)
+ *   diag, first, last : N*N*N_z complex128 each, or NULL when not requested; requested
+ *              outputs are fully written, the t_gfunc components not requested are the
+ *              caller's to zero (src/math_kernel.f90:75-77).
+ */
+int negf_rgf_tri_solve(int N, int N_z, const double *G_inv_D, int mode, double *diag,
+                       double *first, double *last);
+
+/*
+ * Batched equilibrium density (replaces src/main.f90:272-288 over Equilibrium_Density,
+ * src/negf.f90:9-113): for each complex node z[n] (without eta; eta is added) and
+ * complex weight w[n] (the contour weights of src/negf.f90:86-111, k weight applied here):
+ *   density_matrix(:, :, k) += sum_n sum_ik kw[ik] * w[n] * 2*Delta^2 * G_kk(z[n] + i eta)
+ * density_matrix : N*N*N_z complex128, accumulated in place (intent(inout)).
+ */
+int negf_density_batch(negf_ctx *ctx, const double *z, const double *w, int nnode,
+                       const double *kx, const double *ky, const double *kw, int nk,
+                       double *density_matrix);
+
+/*
+ * LDOS for a list of real energies (replaces Local_Density_Of_State,
+ * src/negf.f90:176-236, summed over k): per energy ie
+ *   ldos_matrix(:, :, k, ie) = - sum_ik kw[ik] / pi * 2*Delta^2 * G_kk(E[ie] + i eta)
+ * ldos_matrix : N*N*N_z*nE complex128 (overwritten).
+ */
+int negf_ldos_batch(negf_ctx *ctx, const double *E, int nE, const double *kx,
+                    const double *ky, const double *kw, int nk, double *ldos_matrix);
+
+/*
+ * Plane-wave -> real-space diagonal of nmat N x N blocks
+ * (Greensfunction_PlanewaveToReal, src/grid.f90:71-135):
+ *   out(m_x, m_y, s) = sum_ij M_s(i,j) exp(+2 pi i [(nx_i-nx_j) m_x/N_x + (ny_i-ny_j) m_y/N_y])
+ * mats : N*N*nmat complex128 ; out : N_x*N_y*nmat complex128.
+ */
+int negf_pw_to_real(int N, const int *nx_grid, const int *ny_grid, int nmat,
+                    const double *mats, int N_x, int N_y, double *out);
+
+/*
+ * Real-space potential -> V_reciprocal_all (LocalPotential_RealToPlanewave,
+ * src/grid.f90:12-69, for every slice, plus the window copy of src/main.f90:238-247).
+ *   V_real : N_x*N_y*N_z real (Hartree) ; vrec : (2NGX+1)*(2NGY+1)*N_z complex128 (overwritten)
+ *   full_window : 0 = copy only |p|<=NGX/2, |q|<=NGY/2 (reference), 1 = whole window
+ */
+int negf_vhat_from_real(int N_x, int N_y, int N_z, const double *V_real, int NGX, int NGY,
+                        int full_window, double *vrec);
+
+/*
+ * Batched drop-ins for the two private LAPACK/BLAS wrappers of src/math_kernel.f90:
+ *   negf_inverse_batch        inv[b] = A[b]^{-1}   (inverse, :8-34: zgetrf + zgetri)
+ *   negf_matrix_product_batch C[b] = A[b] B[b]     (matrix_product, :36-42: zgemm 'N','N')
+ * count N x N complex128 matrices, contiguous.  Pivoting follows zgetrf (largest
+ * |re|+|im| in the column, first index on ties).
+ */
+int negf_inverse_batch(int N, int count, const double *A, double *inv);
+int negf_matrix_product_batch(int N, int count, const double *A, const double *B, double *C);
+
+/*
+ * Slice blocks on the GPU, for callers that want them materialised:
+ *   negf_hamiltonian : H(:, :, k) of Hamiltonian_construction (src/grid.f90:236-262)
+ *   negf_e_minus_h   : D_k of E_minus_H_construction (src/grid.f90:264-333) at complex
+ *                      energy (E_re + i E_im) -- eta is NOT added here, as in the reference
+ *                      where the caller passes energy + i*ETA
+ * out : N*N*N_z complex128 (overwritten).  (The batched solvers build the same blocks
+ * on the fly and never store them.)
+ */
+int negf_hamiltonian(negf_ctx *ctx, double kx, double ky, double *out);
+int negf_e_minus_h(negf_ctx *ctx, double E_re, double E_im, double kx, double ky, double *out);
+
+/*
+ * Context-free forms with the argument lists of the reference routines:
+ *   negf_hamiltonian_direct  Hamiltonian_construction(nx, ny, Lx, Ly, kx, ky, V_reciprocal, H)
+ *                            (src/grid.f90:236-262)
+ *   negf_e_minus_h_from_h    E_minus_H_construction(H, energy, nx, ny, Lx, Ly, Lz, kx, ky,
+ *                            V_L, V_R, E_minus_H) (src/grid.f90:264-333), energy complex
+ *   negf_transmission_trace  Transmission_solver on G = first_column(:, :, N_z) already
+ *                            rescaled by 2 Delta^2 (src/grid.f90:335-385); Delta = Lz/N_z
+ * H, out : N*N*N_z complex128 ; G : N*N complex128.
+ */
+int negf_hamiltonian_direct(int N, int N_z, const int *nx_grid, const int *ny_grid, double Lx,
+                            double Ly, double kx, double ky, const double *vrec, int NGX, int NGY,
+                            double *out);
+int negf_e_minus_h_from_h(int N, int N_z, const double *H, double E_re, double E_im,
+                          const int *nx_grid, const int *ny_grid, double Lx, double Ly, double Lz,
+                          double kx, double ky, double V_L, double V_R, double *out);
+int negf_transmission_trace(int N, int N_z, const double *G, double E, const int *nx_grid,
+                            const int *ny_grid, double Lx, double Ly, double Lz, double kx,
+                            double ky, double V_L, double V_R, double *tau);
+
+/* Wall time (seconds) of the last batched call's device work, from HIP events. */
+double negf_last_kernel_seconds(negf_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEGF_GPU_H */

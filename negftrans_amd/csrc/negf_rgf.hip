@@ -1,0 +1,1151 @@
+// negf_rgf.hip -- MI355X (gfx950) implementation of the NEGFtrans RGF energy loop
+// behind the C-ABI declared in include/negf_gpu.h.
+//
+// Hot path: k_transmission, one persistent launch over every (energy, k-point) job.
+// Each workgroup owns one job at a time and runs the whole backward sweep of
+// GreensFunction_tri_solver "F" (src/math_kernel.f90:80-129) with O(N^2) state:
+//     g_{N_z} = D_{N_z}^{-1},  g_m = (D_m - g_{m+1})^{-1},  Q = g_{N_z} g_{N_z-1} ... g_2,
+//     G_{N_z,1} = (-1)^{N_z-1} Q (D_1 - g_2)^{-1}
+// (the reference's generators in the same order; only the product is associated from
+// the right instead of being propagated slice by slice), building each D_m on the fly
+// from the resident V-hat table, and fusing the Caroli trace of Transmission_solver
+// (src/grid.f90:335-385) into the last block product.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/negf_gpu.h"
+#include "rgf_device.h"
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess)                                                          \
+            return fail(-100 - (int)e_, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static const double PI = 3.141592653589793;
+
+// ------------------------------------------------------------------ kernels
+struct TransArgs {
+    SysView sv;
+    int njobs;
+    const cd *z;          // per job E + i eta
+    const int *kidx;      // per job k-point index
+    const double *kinh;   // per k-point, Np each
+    const cd *sigL, *sigR;      // per job, Np each
+    const double *gamL, *gamR;  // per job, Np each (0 for closed channels / padding)
+    cd *ws;               // per slot 3 Np^2
+    double *tau;          // per job
+    int *status;          // per job: first zero pivot (1-based slice) or 0
+};
+
+template <int MAXT>
+__global__ __launch_bounds__(MAXT) void k_transmission(TransArgs a) {
+    __shared__ RgfShared s;
+    const SysView &sv = a.sv;
+    const int Np = sv.Np;
+    const size_t NN = (size_t)Np * Np;
+    cd *buf0 = a.ws + (size_t)blockIdx.x * 3 * NN;
+    for (int job = blockIdx.x; job < a.njobs; job += gridDim.x) {
+        if (threadIdx.x == 0) s.flag = 0;
+        __syncthreads();
+        JobView jv;
+        jv.z = a.z[job];
+        jv.kinh = a.kinh + (size_t)a.kidx[job] * Np;
+        jv.sigL = a.sigL + (size_t)job * Np;
+        jv.sigR = a.sigR + (size_t)job * Np;
+        const double *gL = a.gamL + (size_t)job * Np, *gR = a.gamR + (size_t)job * Np;
+        // S: latest generator (columns permuted by s.sigma), Q: running product, F: free
+        cd *S = buf0, *Q = buf0 + NN, *F = buf0 + 2 * NN;
+        int bad = 0;
+        double part = 0.0;
+        for (int m = sv.Nz; m >= 1; --m) {
+            // M = D_m - g_{m+1} (m = N_z: M = D_{N_z}), inverted in place
+            build_block(F, sv, jv, m - 1, m == sv.Nz ? nullptr : S, s.sigma);
+            __syncthreads();
+            if (gje_invert(F, Np, s) && !bad) bad = m;
+            if (m == sv.Nz) {
+                copy_cols(Q, F, s.sigma, Np);           // Q = g_{N_z}
+                __syncthreads();
+                cd *t = S;
+                S = F;
+                F = t;
+                continue;
+            }
+            // m >= 2: Q <- Q g_m (into the buffer of g_{m+1}, no longer needed);
+            // m == 1: G_{N_z,1} = +-Q G_11, reduced straight into the trace.
+            const bool last = (m == 1);
+            cd *out = S;
+            gemm_nn(Q, F, s.sigma, Np, [&](int r, int c, cd v) {
+                if (last) part += gL[r] * gR[c] * (v.x * v.x + v.y * v.y);
+                else out[r + (size_t)c * Np] = v;
+            });
+            __syncthreads();
+            S = F;
+            F = Q;
+            Q = out;
+        }
+        const double tot = block_sum(part, s);
+        if (threadIdx.x == 0) {
+            a.tau[job] = tot * sv.scale * sv.scale;
+            a.status[job] = bad;
+        }
+        __syncthreads();
+    }
+}
+
+// D mode (diagonal blocks): backward generators stored per slot, then
+//   G_11 = (D_1 - g_2)^{-1},  G_kk = g_k (G_{k-1,k-1} g_k + I)   (src/math_kernel.f90:94-109)
+// and out(:, :, k, tgt) += wgt * G_kk for the N x N physical block.
+struct DiagArgs {
+    SysView sv;
+    int njobs;
+    const cd *z;
+    const int *kidx;
+    const double *kinh;
+    const cd *sigL, *sigR;
+    const cd *wgt;        // per job complex weight (includes 2 Delta^2)
+    const int *target;    // per job output index (density: 0, LDOS: energy index)
+    cd *ws;               // per slot (N_z + 2) Np^2
+    cd *out;              // N*N*N_z per target, column-major ld N
+    int *status;
+};
+
+template <int MAXT>
+__global__ __launch_bounds__(MAXT) void k_diagonal(DiagArgs a) {
+    __shared__ RgfShared s;
+    const SysView &sv = a.sv;
+    const int Np = sv.Np, N = sv.N, Nz = sv.Nz;
+    const size_t NN = (size_t)Np * Np;
+    cd *G = a.ws + (size_t)blockIdx.x * (Nz + 2) * NN;   // G[m] = g_{m+1}, m = 1..Nz-1
+    cd *X = G, *Y = G + (size_t)Nz * NN, *W = G + (size_t)(Nz + 1) * NN;
+    for (int job = blockIdx.x; job < a.njobs; job += gridDim.x) {
+        if (threadIdx.x == 0) s.flag = 0;
+        __syncthreads();
+        JobView jv;
+        jv.z = a.z[job];
+        jv.kinh = a.kinh + (size_t)a.kidx[job] * Np;
+        jv.sigL = a.sigL + (size_t)job * Np;
+        jv.sigR = a.sigR + (size_t)job * Np;
+        const cd wgt = a.wgt[job];
+        cd *out = a.out + (size_t)a.target[job] * N * N * Nz;
+        int bad = 0;
+        // backward generators g_{N_z} .. g_2 stored unpermuted; the last pass leaves
+        // G_11 = (D_1 - g_2)^{-1} in X
+        for (int m = Nz; m >= 1; --m) {
+            build_block(W, sv, jv, m - 1, m == Nz ? nullptr : G + (size_t)m * NN, nullptr);
+            __syncthreads();
+            if (gje_invert(W, Np, s) && !bad) bad = m;
+            copy_cols(m >= 2 ? G + (size_t)(m - 1) * NN : X, W, s.sigma, Np);
+            __syncthreads();
+        }
+        for (int col = 0; col < N; ++col)
+            for (int r = threadIdx.x; r < N; r += blockDim.x) {
+                const cd v = cmul(wgt, X[r + (size_t)col * Np]);
+                double *o = reinterpret_cast<double *>(out + r + (size_t)col * N);
+                atomicAdd(o, v.x);
+                atomicAdd(o + 1, v.y);
+            }
+        __syncthreads();
+        // G_kk = g_k (G_{k-1,k-1} g_k + I)  (two products per slice, one call site)
+        for (int step = 2; step < 2 * Nz; ++step) {
+            const int k = step >> 1;          // 0-based slice, generator g_{k+1} at G[k]
+            const bool second = step & 1;
+            const cd *gk = G + (size_t)k * NN;
+            cd *ok = out + (size_t)k * N * N;
+            gemm_nn(second ? gk : X, second ? Y : gk, nullptr, Np, [&](int r, int c, cd v) {
+                if (!second) {
+                    if (r == c) v.x += 1.0;
+                    Y[r + (size_t)c * Np] = v;
+                } else {
+                    X[r + (size_t)c * Np] = v;
+                    if (r < N && c < N) {
+                        const cd t = cmul(wgt, v);
+                        double *o = reinterpret_cast<double *>(ok + r + (size_t)c * N);
+                        atomicAdd(o, t.x);
+                        atomicAdd(o + 1, t.y);
+                    }
+                }
+            });
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) a.status[job] = bad;
+        __syncthreads();
+    }
+}
+
+// Batched primitives (drop-ins for inverse / matrix_product, src/math_kernel.f90:8-42,
+// and the building blocks of the generic tri-diagonal solver).
+// inv[b] = (A[b] - (Sub ? Sub[b] : 0))^{-1} for count matrices of N x N (ld N on both
+// sides); the work copy is padded to Np.
+template <int MAXT>
+__global__ __launch_bounds__(MAXT) void k_inverse_batch(int N, int Np, const cd *A, const cd *Sub,
+                                                        cd *work, cd *inv, int *status) {
+    __shared__ RgfShared s;
+    const int b = blockIdx.x;
+    const size_t NN = (size_t)Np * Np;
+    cd *Wk = work + b * NN;
+    const cd *Ab = A + (size_t)b * N * N;
+    const cd *Sb = Sub ? Sub + (size_t)b * N * N : nullptr;
+    if (threadIdx.x == 0) s.flag = 0;
+    for (int col = 0; col < Np; ++col)
+        for (int r = threadIdx.x; r < Np; r += blockDim.x) {
+            cd v = cmk(r == col ? 1.0 : 0.0, 0.0);
+            if (r < N && col < N) {
+                v = Ab[r + (size_t)col * N];
+                if (Sb) v = csub(v, Sb[r + (size_t)col * N]);
+            }
+            Wk[r + (size_t)col * Np] = v;
+        }
+    __syncthreads();
+    const int zp = gje_invert(Wk, Np, s);
+    cd *ob = inv + (size_t)b * N * N;
+    for (int col = 0; col < N; ++col) {
+        const cd *sc = Wk + (size_t)s.sigma[col] * Np;
+        for (int r = threadIdx.x; r < N; r += blockDim.x) ob[r + (size_t)col * N] = sc[r];
+    }
+    if (threadIdx.x == 0) status[b] = zp;
+}
+
+// C[b] = alpha * A[ia(b)] * B[ib(b)] (+ I if add_identity), N x N, ld N; operands are
+// padded into the work area so every product runs at the padded size Np.
+template <int MAXT>
+__global__ __launch_bounds__(MAXT) void k_gemm_batch(int N, int Np, const cd *A, long strideA,
+                                                     const cd *B, long strideB, cd *C,
+                                                     long strideC, double alpha, int add_identity,
+                                                     cd *work) {
+    const int b = blockIdx.x;
+    const size_t NN = (size_t)Np * Np;
+    cd *Ap = work + (size_t)b * 2 * NN, *Bp = Ap + NN;
+    const cd *Ab = A + b * strideA, *Bb = B + b * strideB;
+    for (int col = 0; col < Np; ++col)
+        for (int r = threadIdx.x; r < Np; r += blockDim.x) {
+            const bool in = r < N && col < N;
+            Ap[r + (size_t)col * Np] = in ? Ab[r + (size_t)col * N] : cmk(0.0, 0.0);
+            Bp[r + (size_t)col * Np] = in ? Bb[r + (size_t)col * N] : cmk(0.0, 0.0);
+        }
+    __syncthreads();
+    cd *Cb = C + b * strideC;
+    gemm_nn(Ap, Bp, nullptr, Np, [&](int r, int c, cd v) {
+        if (r < N && c < N) {
+            v.x *= alpha;
+            v.y *= alpha;
+            if (add_identity && r == c) v.x += 1.0;
+            Cb[r + (size_t)c * N] = v;
+        }
+    });
+}
+
+// Plane-wave -> real space (src/grid.f90:71-135) by the equivalent Q-grid form:
+// rho(Q) = sum_{n_i - n_j = Q mod grid} M_ij, out = unnormalised inverse 2-D DFT of rho.
+// One workgroup per matrix; rho and the half-transformed grid live in global scratch.
+__global__ __launch_bounds__(256) void k_pw_to_real(int N, const int *nx, const int *ny,
+                                                    const cd *mats, int Nx, int Ny,
+                                                    const cd *twx, const cd *twy, cd *scratch,
+                                                    cd *out) {
+    const int b = blockIdx.x;
+    const cd *M = mats + (size_t)b * N * N;
+    cd *rho = scratch + (size_t)b * 2 * Nx * Ny, *half = rho + (size_t)Nx * Ny;
+    for (int i = threadIdx.x; i < Nx * Ny; i += blockDim.x) rho[i] = cmk(0.0, 0.0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        const int i = e % N, j = e / N;
+        int qx = (nx[i] - nx[j]) % Nx, qy = (ny[i] - ny[j]) % Ny;
+        if (qx < 0) qx += Nx;
+        if (qy < 0) qy += Ny;
+        const cd v = M[e];
+        double *o = reinterpret_cast<double *>(rho + qx + (size_t)qy * Nx);
+        atomicAdd(o, v.x);
+        atomicAdd(o + 1, v.y);
+    }
+    __syncthreads();
+    // half(mx, qy) = sum_qx rho(qx, qy) e^{+2 pi i qx mx / Nx}
+    for (int e = threadIdx.x; e < Nx * Ny; e += blockDim.x) {
+        const int mx = e % Nx, qy = e / Nx;
+        cd acc = cmk(0.0, 0.0);
+        for (int qx = 0; qx < Nx; ++qx) acc = cfma(rho[qx + (size_t)qy * Nx], twx[(qx * mx) % Nx], acc);
+        half[e] = acc;
+    }
+    __syncthreads();
+    cd *ob = out + (size_t)b * Nx * Ny;
+    for (int e = threadIdx.x; e < Nx * Ny; e += blockDim.x) {
+        const int mx = e % Nx, my = e / Nx;
+        cd acc = cmk(0.0, 0.0);
+        for (int qy = 0; qy < Ny; ++qy) acc = cfma(half[mx + (size_t)qy * Nx], twy[(qy * my) % Ny], acc);
+        ob[e] = acc;
+    }
+}
+
+// V-hat of every slice (src/grid.f90:12-69 + src/main.f90:238-247):
+//   V(p, q) = (1/(Nx Ny)) sum_{x,y} V(x, y) e^{-2 pi i (p x/Nx + q y/Ny)}, p = 0..NGX,
+// p < 0 by conjugate symmetry, copied into the window |p| <= hx, |q| <= hy (zero elsewhere).
+// One workgroup per slice; twx/twy hold e^{-2 pi i j/N}.
+__global__ __launch_bounds__(256) void k_vhat(int Nx, int Ny, const double *V, int NGX, int NGY,
+                                              int hx, int hy, const cd *twx, const cd *twy,
+                                              cd *scratch, cd *vrec) {
+    const int k = blockIdx.x;
+    const double *Vs = V + (size_t)k * Nx * Ny;
+    cd *A = scratch + (size_t)k * (NGX + 1) * Ny;   // A(p, y) = sum_x V(x,y) e^{-2 pi i p x/Nx}
+    for (int e = threadIdx.x; e < (NGX + 1) * Ny; e += blockDim.x) {
+        const int p = e % (NGX + 1), y = e / (NGX + 1);
+        cd acc = cmk(0.0, 0.0);
+        for (int x = 0; x < Nx; ++x) {
+            const cd t = twx[(p * x) % Nx];
+            const double v = Vs[x + (size_t)y * Nx];
+            acc.x = fma(v, t.x, acc.x);
+            acc.y = fma(v, t.y, acc.y);
+        }
+        A[e] = acc;
+    }
+    __syncthreads();
+    const int W = 2 * NGX + 1, H = 2 * NGY + 1;
+    cd *out = vrec + (size_t)k * W * H;
+    const double norm = 1.0 / ((double)Nx * Ny);
+    for (int e = threadIdx.x; e < W * H; e += blockDim.x) {
+        const int p = e % W - NGX, q = e / W - NGY;
+        cd v = cmk(0.0, 0.0);
+        if (p >= -hx && p <= hx && q >= -hy && q <= hy) {
+            const int pp = p >= 0 ? p : -p, qq = p >= 0 ? q : -q;
+            const int qm = ((qq % Ny) + Ny) % Ny;
+            cd acc = cmk(0.0, 0.0);
+            for (int y = 0; y < Ny; ++y) acc = cfma(A[pp + (size_t)y * (NGX + 1)], twy[(qm * y) % Ny], acc);
+            v = cmk(acc.x * norm, (p >= 0 ? acc.y : -acc.y) * norm);
+        }
+        out[e] = v;
+    }
+}
+
+// Materialised slice blocks: one workgroup per slice.  mode 0: H_k (src/grid.f90:248-259);
+// mode 1: D_k (src/grid.f90:291-331).  out is N x N per slice (ld N).
+__global__ __launch_bounds__(256) void k_blocks(SysView sv, JobView jv, int mode, cd *work, cd *out) {
+    const int k = blockIdx.x, N = sv.N, Np = sv.Np;
+    const int W = 2 * sv.NGX + 1;
+    const cd *vs = sv.vrec + (size_t)k * W * (2 * sv.NGY + 1);
+    cd *ob = out + (size_t)k * N * N;
+    if (mode == 1) {
+        cd *M = work + (size_t)k * Np * Np;
+        build_block(M, sv, jv, k, nullptr, nullptr);
+        __syncthreads();
+        for (int e = threadIdx.x; e < N * N; e += blockDim.x) ob[e] = M[(e % N) + (size_t)(e / N) * Np];
+        return;
+    }
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        const int r = e % N, c = e / N;
+        cd h = vs[(sv.nx[r] - sv.nx[c] + sv.NGX) + W * (sv.ny[r] - sv.ny[c] + sv.NGY)];
+        if (r == c) h.x += jv.kinh[r];
+        ob[e] = h;
+    }
+}
+
+// D_k from a given H (E_minus_H_construction's own argument list): one workgroup per
+// slice; lead terms (precomputed per basis function) on the first and last slice.
+__global__ __launch_bounds__(256) void k_emh_from_h(int N, int Nz, const cd *H, cd z, double scale,
+                                                    const cd *sigL, const cd *sigR, cd *out) {
+    const int k = blockIdx.x;
+    const cd *Hk = H + (size_t)k * N * N;
+    cd *ok = out + (size_t)k * N * N;
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        const int r = e % N, c = e / N;
+        cd d = cmk(-Hk[e].x, -Hk[e].y);
+        if (r == c) d = cadd(d, z);
+        d.x *= scale;
+        d.y *= scale;
+        if (r == c) {
+            d.x -= 2.0;
+            if (k == 0) d = cadd(d, sigL[r]);
+            if (k == Nz - 1) d = cadd(d, sigR[r]);
+        }
+        ok[e] = d;
+    }
+}
+
+// tau = sum_ij gL_i gR_j |G_ij|^2 over one N x N block (single workgroup).
+__global__ __launch_bounds__(256) void k_trace(int N, const cd *G, const double *gL, const double *gR,
+                                               double *tau) {
+    __shared__ double red[4];
+    double part = 0.0;
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        const int r = e % N, c = e / N;
+        part += gL[r] * gR[c] * (G[e].x * G[e].x + G[e].y * G[e].y);
+    }
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) *tau = red[0] + red[1] + red[2] + red[3];
+}
+
+// ------------------------------------------------------------------ host side
+struct negf_ctx {
+    int device = 0;
+    int N = 0, Nz = 0, Np = 0, NGX = 0, NGY = 0, flags = 0;
+    double Lx = 0, Ly = 0, Lz = 0, VL = 0, VR = 0, eta = 0;
+    std::vector<int> nx, ny;
+    int *d_nx = nullptr, *d_ny = nullptr;
+    cd *d_vrec = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_kernel_s = 0.0;
+    int n_cu = 0;
+    // reusable device buffers
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+};
+
+namespace {
+
+int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+template <class T>
+int dmalloc(T **p, size_t count) {
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) return fail(-100 - (int)e, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int ensure_ws(negf_ctx *c, size_t bytes) {
+    if (c->ws_bytes >= bytes) return 0;
+    if (c->ws) HIPCHK(hipFree(c->ws));
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+    HIPCHK(hipMalloc(&c->ws, bytes));
+    c->ws_bytes = bytes;
+    return 0;
+}
+
+// RAII list of temporary device allocations.
+struct Temps {
+    std::vector<void *> ptrs;
+    ~Temps() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    int alloc(T **p, size_t count) {
+        int st = dmalloc(p, count);
+        if (st == 0) ptrs.push_back((void *)*p);
+        return st;
+    }
+    template <class T>
+    int upload(T **p, const T *host, size_t count, hipStream_t s) {
+        int st = alloc(p, count);
+        if (st) return st;
+        hipError_t e = hipMemcpyAsync(*p, host, count * sizeof(T), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return fail(-100 - (int)e, std::string("upload: ") + hipGetErrorString(e));
+        return 0;
+    }
+};
+
+int threads_for(int Np) { return std::min(1024, round_up(Np, 64)); }
+
+// Per-job host tables: kinetic/2 per k-point, lead terms and trace weights per job.
+struct JobTables {
+    std::vector<cd> z, sigL, sigR;
+    std::vector<double> kinh, gamL, gamR;
+    std::vector<int> kidx;
+};
+
+void job_tables(const negf_ctx *c, const std::vector<std::complex<double>> &energies,
+                const std::vector<double> &realE, const double *kx, const double *ky, int nk,
+                const std::vector<int> &job_e, const std::vector<int> &job_k, bool with_gamma,
+                JobTables &t) {
+    const int Np = c->Np, N = c->N;
+    const double delta = c->Lz / c->Nz;
+    t.kinh.assign((size_t)nk * Np, 0.0);
+    std::vector<double> kin((size_t)nk * N);
+    for (int ik = 0; ik < nk; ++ik)
+        for (int i = 0; i < N; ++i) {
+            const double gx = 2 * c->nx[i] * PI / c->Lx + kx[ik];
+            const double gy = 2 * c->ny[i] * PI / c->Ly + ky[ik];
+            kin[(size_t)ik * N + i] = gx * gx + gy * gy;
+            t.kinh[(size_t)ik * Np + i] = (gx * gx + gy * gy) / 2;
+        }
+    const size_t nj = job_e.size();
+    t.z.resize(nj);
+    t.kidx.resize(nj);
+    t.sigL.assign(nj * Np, make_double2(0, 0));
+    t.sigR.assign(nj * Np, make_double2(0, 0));
+    if (with_gamma) {
+        t.gamL.assign(nj * Np, 0.0);
+        t.gamR.assign(nj * Np, 0.0);
+    }
+    for (size_t j = 0; j < nj; ++j) {
+        const std::complex<double> E = energies[job_e[j]] + std::complex<double>(0.0, c->eta);
+        t.z[j] = make_double2(E.real(), E.imag());
+        t.kidx[j] = job_k[j];
+        for (int i = 0; i < N; ++i) {
+            const double kn = kin[(size_t)job_k[j] * N + i];
+            // src/grid.f90:284-289 (principal sqrt) and :323-331
+            const std::complex<double> qR = std::sqrt(2.0 * (E - c->VL) - kn);
+            const std::complex<double> qT = std::sqrt(2.0 * (E - c->VR) - kn);
+            const std::complex<double> sl = std::exp(std::complex<double>(0, 1) * qR * delta);
+            const std::complex<double> sr = std::exp(std::complex<double>(0, 1) * qT * delta);
+            t.sigL[j * Np + i] = make_double2(sl.real(), sl.imag());
+            t.sigR[j * Np + i] = make_double2(sr.real(), sr.imag());
+            if (with_gamma) {
+                // src/grid.f90:355-360,365,372: real E, closed channels clamped to 0
+                const double Er = realE[job_e[j]];
+                const double QR = std::sqrt(std::max(2 * (Er - c->VL) - kn, 0.0));
+                const double QT = std::sqrt(std::max(2 * (Er - c->VR) - kn, 0.0));
+                t.gamL[j * Np + i] = std::sin(QR * delta) / (delta * delta);
+                t.gamR[j * Np + i] = std::sin(QT * delta) / (delta * delta);
+            }
+        }
+    }
+}
+
+int check_ctx(const negf_ctx *c) {
+    if (!c) return fail(-1, "null context");
+    return 0;
+}
+
+SysView sysview(const negf_ctx *c) {
+    SysView sv;
+    sv.N = c->N;
+    sv.Nz = c->Nz;
+    sv.Np = c->Np;
+    sv.NGX = c->NGX;
+    sv.NGY = c->NGY;
+    const double delta = c->Lz / c->Nz;
+    sv.scale = 2.0 * delta * delta;
+    sv.nx = c->d_nx;
+    sv.ny = c->d_ny;
+    sv.vrec = c->d_vrec;
+    return sv;
+}
+
+int slots_for(negf_ctx *c, const void *kernel, int threads, int njobs, size_t per_slot_bytes) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    long slots = (long)c->n_cu * per_cu;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && per_slot_bytes > 0) {
+        const long fit = (long)((free_b * 0.8) / per_slot_bytes);
+        slots = std::min(slots, std::max(1L, fit));
+    }
+    return (int)std::max(1L, std::min<long>(slots, njobs));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *negf_version(void) { return "negf_gpu 0.1 (gfx950, FP64 RGF)"; }
+const char *negf_last_error(void) { return g_err.c_str(); }
+
+int negf_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int negf_ctx_create(negf_ctx **out, int N, int N_z, const int *nx_grid, const int *ny_grid,
+                    double Lx, double Ly, double Lz, double V_L, double V_R, double eta,
+                    const double *vrec, int NGX, int NGY, int flags) {
+    if (!out) return fail(-1, "out is null");
+    *out = nullptr;
+    if (N < 1 || N_z < 2) return fail(-2, "need N >= 1 and N_z >= 2");
+    if (!nx_grid || !ny_grid || !vrec) return fail(-3, "null array argument");
+    if (NGX < 0 || NGY < 0) return fail(-4, "NGX/NGY must be >= 0");
+    const int Np = round_up(N, 16);
+    if (Np > NEGF_MAXNP) return fail(-5, "N too large for one workgroup (N <= 1024)");
+    for (int i = 0; i < N; ++i) {
+        for (int j = 0; j < N; ++j) {
+            if (std::abs(nx_grid[i] - nx_grid[j]) > NGX || std::abs(ny_grid[i] - ny_grid[j]) > NGY)
+                return fail(-6, "basis differences exceed the V_reciprocal window (NGX/NGY too small)");
+        }
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-7, "no GPU visible");
+    auto *c = new negf_ctx();
+    c->N = N;
+    c->Nz = N_z;
+    c->Np = Np;
+    c->NGX = NGX;
+    c->NGY = NGY;
+    c->flags = flags;
+    c->Lx = Lx;
+    c->Ly = Ly;
+    c->Lz = Lz;
+    c->VL = V_L;
+    c->VR = V_R;
+    c->eta = eta;
+    c->nx.assign(nx_grid, nx_grid + N);
+    c->ny.assign(ny_grid, ny_grid + N);
+    auto bail = [&](int st) {
+        negf_ctx_destroy(c);
+        return st;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(-8, "hipGetDevice failed"));
+    if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
+        return bail(fail(-8, "device attribute query failed"));
+    if (hipStreamCreate(&c->stream) != hipSuccess) return bail(fail(-9, "hipStreamCreate failed"));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return bail(fail(-9, "hipEventCreate failed"));
+    const size_t nv = (size_t)(2 * NGX + 1) * (2 * NGY + 1) * N_z;
+    int st;
+    if ((st = dmalloc(&c->d_nx, N)) || (st = dmalloc(&c->d_ny, N)) || (st = dmalloc(&c->d_vrec, nv)))
+        return bail(st);
+    if (hipMemcpy(c->d_nx, nx_grid, N * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_ny, ny_grid, N * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_vrec, vrec, nv * sizeof(cd), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(-10, "upload of basis / V_reciprocal failed"));
+    *out = c;
+    return 0;
+}
+
+int negf_ctx_destroy(negf_ctx *c) {
+    if (!c) return 0;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->d_nx) (void)hipFree(c->d_nx);
+    if (c->d_ny) (void)hipFree(c->d_ny);
+    if (c->d_vrec) (void)hipFree(c->d_vrec);
+    if (c->ws) (void)hipFree(c->ws);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+double negf_last_kernel_seconds(negf_ctx *c) { return c ? c->last_kernel_s : -1.0; }
+
+int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *kx,
+                            const double *ky, const double *kw, int nk, double *tau,
+                            double *tau_jobs) {
+    int st = check_ctx(c);
+    if (st) return st;
+    if (nE < 0 || nk < 1 || (nE > 0 && (!E || !tau)) || !kx || !ky || !kw)
+        return fail(-2, "bad arguments to negf_transmission_batch");
+    if (nE == 0) return 0;
+    HIPCHK(hipSetDevice(c->device));
+    const int nj = nE * nk;
+    std::vector<std::complex<double>> energies(nE);
+    std::vector<double> realE(E, E + nE);
+    std::vector<int> je(nj), jk(nj);
+    for (int i = 0; i < nE; ++i) energies[i] = E[i];
+    for (int j = 0; j < nj; ++j) {
+        je[j] = j % nE;   // i_energy = 1 + mod(i_job - 1, N_E)   (src/negf.f90:132-133)
+        jk[j] = j / nE;
+    }
+    JobTables t;
+    job_tables(c, energies, realE, kx, ky, nk, je, jk, true, t);
+    Temps tmp;
+    TransArgs a;
+    a.sv = sysview(c);
+    a.njobs = nj;
+    cd *d_z, *d_sl, *d_sr;
+    int *d_k, *d_st;
+    double *d_kin, *d_gl, *d_gr, *d_tau;
+    const size_t njp = (size_t)nj * c->Np;
+    if ((st = tmp.upload(&d_z, t.z.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_k, t.kidx.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_kin, t.kinh.data(), (size_t)nk * c->Np, c->stream)) ||
+        (st = tmp.upload(&d_sl, t.sigL.data(), njp, c->stream)) ||
+        (st = tmp.upload(&d_sr, t.sigR.data(), njp, c->stream)) ||
+        (st = tmp.upload(&d_gl, t.gamL.data(), njp, c->stream)) ||
+        (st = tmp.upload(&d_gr, t.gamR.data(), njp, c->stream)) ||
+        (st = tmp.alloc(&d_tau, nj)) || (st = tmp.alloc(&d_st, nj)))
+        return st;
+    const int threads = threads_for(c->Np);
+    const size_t per_slot = (size_t)3 * c->Np * c->Np * sizeof(cd);
+    const void *kfn = threads <= 512 ? (const void *)k_transmission<512> : (const void *)k_transmission<1024>;
+    const int slots = slots_for(c, kfn, threads, nj, per_slot);
+    if ((st = ensure_ws(c, per_slot * slots))) return st;
+    a.z = d_z;
+    a.kidx = d_k;
+    a.kinh = d_kin;
+    a.sigL = d_sl;
+    a.sigR = d_sr;
+    a.gamL = d_gl;
+    a.gamR = d_gr;
+    a.ws = (cd *)c->ws;
+    a.tau = d_tau;
+    a.status = d_st;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if (threads <= 512) hipLaunchKernelGGL(k_transmission<512>, dim3(slots), dim3(threads), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_transmission<1024>, dim3(slots), dim3(threads), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    std::vector<double> h_tau(nj);
+    std::vector<int> h_st(nj);
+    HIPCHK(hipMemcpyAsync(h_tau.data(), d_tau, nj * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(h_st.data(), d_st, nj * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_kernel_s = ms * 1e-3;
+    int status = 0;
+    for (int j = 0; j < nj; ++j) {
+        tau[je[j]] += h_tau[j] * kw[jk[j]];   // src/negf.f90:172
+        if (tau_jobs) tau_jobs[j] = h_tau[j];
+        if (h_st[j] && !status) status = h_st[j];
+    }
+    return status;
+}
+
+static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &energies,
+                         const std::vector<int> &je, const std::vector<int> &jk,
+                         const std::vector<std::complex<double>> &wgt,
+                         const std::vector<int> &target, int ntarget, const double *kx,
+                         const double *ky, int nk, double *host_out, bool accumulate) {
+    int st;
+    HIPCHK(hipSetDevice(c->device));
+    const int nj = (int)je.size();
+    std::vector<double> realE(energies.size());
+    for (size_t i = 0; i < energies.size(); ++i) realE[i] = energies[i].real();
+    JobTables t;
+    job_tables(c, energies, realE, kx, ky, nk, je, jk, false, t);
+    Temps tmp;
+    DiagArgs a;
+    a.sv = sysview(c);
+    a.njobs = nj;
+    cd *d_z, *d_sl, *d_sr, *d_w, *d_out;
+    int *d_k, *d_t, *d_st;
+    double *d_kin;
+    const size_t njp = (size_t)nj * c->Np;
+    const size_t nout = (size_t)ntarget * c->N * c->N * c->Nz;
+    std::vector<cd> hw(nj);
+    for (int j = 0; j < nj; ++j) hw[j] = make_double2(wgt[j].real(), wgt[j].imag());
+    if ((st = tmp.upload(&d_z, t.z.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_k, t.kidx.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_kin, t.kinh.data(), (size_t)nk * c->Np, c->stream)) ||
+        (st = tmp.upload(&d_sl, t.sigL.data(), njp, c->stream)) ||
+        (st = tmp.upload(&d_sr, t.sigR.data(), njp, c->stream)) ||
+        (st = tmp.upload(&d_w, hw.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_t, target.data(), nj, c->stream)) || (st = tmp.alloc(&d_st, nj)))
+        return st;
+    if (accumulate) {
+        if ((st = tmp.upload(&d_out, reinterpret_cast<const cd *>(host_out), nout, c->stream))) return st;
+    } else {
+        if ((st = tmp.alloc(&d_out, nout))) return st;
+        HIPCHK(hipMemsetAsync(d_out, 0, nout * sizeof(cd), c->stream));
+    }
+    const int threads = threads_for(c->Np);
+    const size_t per_slot = (size_t)(c->Nz + 2) * c->Np * c->Np * sizeof(cd);
+    const void *kfn = threads <= 512 ? (const void *)k_diagonal<512> : (const void *)k_diagonal<1024>;
+    const int slots = slots_for(c, kfn, threads, nj, per_slot);
+    if ((st = ensure_ws(c, per_slot * slots))) return st;
+    a.z = d_z;
+    a.kidx = d_k;
+    a.kinh = d_kin;
+    a.sigL = d_sl;
+    a.sigR = d_sr;
+    a.wgt = d_w;
+    a.target = d_t;
+    a.ws = (cd *)c->ws;
+    a.out = d_out;
+    a.status = d_st;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if (threads <= 512) hipLaunchKernelGGL(k_diagonal<512>, dim3(slots), dim3(threads), 0, c->stream, a);
+    else hipLaunchKernelGGL(k_diagonal<1024>, dim3(slots), dim3(threads), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    std::vector<int> h_st(nj);
+    HIPCHK(hipMemcpyAsync(host_out, d_out, nout * sizeof(cd), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(h_st.data(), d_st, nj * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_kernel_s = ms * 1e-3;
+    for (int j = 0; j < nj; ++j)
+        if (h_st[j]) return h_st[j];
+    return 0;
+}
+
+static int blocks_common(negf_ctx *c, int mode, std::complex<double> E, double kx, double ky,
+                         double *out) {
+    int st = check_ctx(c);
+    if (st) return st;
+    if (!out) return fail(-2, "out is null");
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<std::complex<double>> en{E};
+    std::vector<double> re{E.real()};
+    std::vector<int> je{0}, jk{0};
+    JobTables t;
+    job_tables(c, en, re, &kx, &ky, 1, je, jk, false, t);
+    // job_tables adds +i*eta; the block builders take the energy as given
+    t.z[0] = make_double2(E.real(), E.imag());
+    {
+        const double delta = c->Lz / c->Nz;
+        for (int i = 0; i < c->N; ++i) {
+            const double gx = 2 * c->nx[i] * PI / c->Lx + kx, gy = 2 * c->ny[i] * PI / c->Ly + ky;
+            const double kn = gx * gx + gy * gy;
+            const std::complex<double> qR = std::sqrt(2.0 * (E - c->VL) - kn);
+            const std::complex<double> qT = std::sqrt(2.0 * (E - c->VR) - kn);
+            const std::complex<double> sl = std::exp(std::complex<double>(0, 1) * qR * delta);
+            const std::complex<double> sr = std::exp(std::complex<double>(0, 1) * qT * delta);
+            t.sigL[i] = make_double2(sl.real(), sl.imag());
+            t.sigR[i] = make_double2(sr.real(), sr.imag());
+        }
+    }
+    Temps tmp;
+    double *d_kin;
+    cd *d_sl, *d_sr, *d_w = nullptr, *d_out;
+    const size_t nout = (size_t)c->N * c->N * c->Nz;
+    if ((st = tmp.upload(&d_kin, t.kinh.data(), (size_t)c->Np, c->stream)) ||
+        (st = tmp.upload(&d_sl, t.sigL.data(), (size_t)c->Np, c->stream)) ||
+        (st = tmp.upload(&d_sr, t.sigR.data(), (size_t)c->Np, c->stream)) ||
+        (st = tmp.alloc(&d_out, nout)))
+        return st;
+    if (mode == 1 && (st = tmp.alloc(&d_w, (size_t)c->Np * c->Np * c->Nz))) return st;
+    JobView jv;
+    jv.z = t.z[0];
+    jv.kinh = d_kin;
+    jv.sigL = d_sl;
+    jv.sigR = d_sr;
+    hipLaunchKernelGGL(k_blocks, dim3(c->Nz), dim3(256), 0, c->stream, sysview(c), jv, mode, d_w, d_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, d_out, nout * sizeof(cd), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int negf_hamiltonian(negf_ctx *c, double kx, double ky, double *out) {
+    return blocks_common(c, 0, 0.0, kx, ky, out);
+}
+
+int negf_e_minus_h(negf_ctx *c, double E_re, double E_im, double kx, double ky, double *out) {
+    return blocks_common(c, 1, std::complex<double>(E_re, E_im), kx, ky, out);
+}
+
+int negf_hamiltonian_direct(int N, int N_z, const int *nx_grid, const int *ny_grid, double Lx,
+                            double Ly, double kx, double ky, const double *vrec, int NGX, int NGY,
+                            double *out) {
+    negf_ctx *c = nullptr;
+    int st = negf_ctx_create(&c, N, N_z, nx_grid, ny_grid, Lx, Ly, 1.0, 0.0, 0.0, 0.0, vrec, NGX, NGY, 0);
+    if (st) return st;
+    st = negf_hamiltonian(c, kx, ky, out);
+    negf_ctx_destroy(c);
+    return st;
+}
+
+// exp(i q Delta) and sin(Q Delta)/Delta^2 per basis function (src/grid.f90:284-289, 354-360)
+static void lead_terms(int N, const int *nx, const int *ny, double Lx, double Ly, double delta,
+                       double kx, double ky, double VL, double VR, std::complex<double> E,
+                       std::vector<cd> *sl, std::vector<cd> *sr, std::vector<double> *gl,
+                       std::vector<double> *gr) {
+    for (int i = 0; i < N; ++i) {
+        const double gx = 2 * nx[i] * PI / Lx + kx, gy = 2 * ny[i] * PI / Ly + ky;
+        const double kn = gx * gx + gy * gy;
+        if (sl) {
+            const std::complex<double> I(0, 1);
+            const std::complex<double> a = std::exp(I * std::sqrt(2.0 * (E - VL) - kn) * delta);
+            const std::complex<double> b = std::exp(I * std::sqrt(2.0 * (E - VR) - kn) * delta);
+            (*sl)[i] = make_double2(a.real(), a.imag());
+            (*sr)[i] = make_double2(b.real(), b.imag());
+        }
+        if (gl) {
+            const double Er = E.real();
+            (*gl)[i] = std::sin(std::sqrt(std::max(2 * (Er - VL) - kn, 0.0)) * delta) / (delta * delta);
+            (*gr)[i] = std::sin(std::sqrt(std::max(2 * (Er - VR) - kn, 0.0)) * delta) / (delta * delta);
+        }
+    }
+}
+
+int negf_e_minus_h_from_h(int N, int N_z, const double *H, double E_re, double E_im,
+                          const int *nx_grid, const int *ny_grid, double Lx, double Ly, double Lz,
+                          double kx, double ky, double V_L, double V_R, double *out) {
+    if (N < 1 || N_z < 1 || !H || !out || !nx_grid || !ny_grid) return fail(-2, "bad arguments to negf_e_minus_h_from_h");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-7, "no GPU visible");
+    const double delta = Lz / N_z;
+    std::vector<cd> sl(N), sr(N);
+    lead_terms(N, nx_grid, ny_grid, Lx, Ly, delta, kx, ky, V_L, V_R, std::complex<double>(E_re, E_im),
+               &sl, &sr, nullptr, nullptr);
+    Temps tmp;
+    cd *dH, *dO, *dl, *dr;
+    int st;
+    const size_t n = (size_t)N * N * N_z;
+    if ((st = tmp.upload(&dH, reinterpret_cast<const cd *>(H), n, nullptr)) || (st = tmp.alloc(&dO, n)) ||
+        (st = tmp.upload(&dl, sl.data(), N, nullptr)) || (st = tmp.upload(&dr, sr.data(), N, nullptr)))
+        return st;
+    hipLaunchKernelGGL(k_emh_from_h, dim3(N_z), dim3(256), 0, nullptr, N, N_z, dH, make_double2(E_re, E_im),
+                       2.0 * delta * delta, dl, dr, dO);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out, dO, n * sizeof(cd), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int negf_transmission_trace(int N, int N_z, const double *G, double E, const int *nx_grid,
+                            const int *ny_grid, double Lx, double Ly, double Lz, double kx,
+                            double ky, double V_L, double V_R, double *tau) {
+    if (N < 1 || N_z < 1 || !G || !tau || !nx_grid || !ny_grid) return fail(-2, "bad arguments to negf_transmission_trace");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-7, "no GPU visible");
+    std::vector<double> gl(N), gr(N);
+    lead_terms(N, nx_grid, ny_grid, Lx, Ly, Lz / N_z, kx, ky, V_L, V_R, std::complex<double>(E, 0.0),
+               nullptr, nullptr, &gl, &gr);
+    Temps tmp;
+    cd *dG;
+    double *dl, *dr, *dt;
+    int st;
+    if ((st = tmp.upload(&dG, reinterpret_cast<const cd *>(G), (size_t)N * N, nullptr)) ||
+        (st = tmp.upload(&dl, gl.data(), N, nullptr)) || (st = tmp.upload(&dr, gr.data(), N, nullptr)) ||
+        (st = tmp.alloc(&dt, 1)))
+        return st;
+    hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, nullptr, N, dG, dl, dr, dt);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(tau, dt, sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int negf_density_batch(negf_ctx *c, const double *z, const double *w, int nnode, const double *kx,
+                       const double *ky, const double *kw, int nk, double *density_matrix) {
+    int st = check_ctx(c);
+    if (st) return st;
+    if (nnode < 0 || nk < 1 || !kx || !ky || !kw || (nnode > 0 && (!z || !w || !density_matrix)))
+        return fail(-2, "bad arguments to negf_density_batch");
+    if (nnode == 0) return 0;
+    const double delta = c->Lz / c->Nz, s = 2 * delta * delta;
+    std::vector<std::complex<double>> en(nnode), wg;
+    std::vector<int> je, jk, tg;
+    for (int n = 0; n < nnode; ++n) en[n] = std::complex<double>(z[2 * n], z[2 * n + 1]);
+    for (int ik = 0; ik < nk; ++ik)
+        for (int n = 0; n < nnode; ++n) {
+            je.push_back(n);
+            jk.push_back(ik);
+            tg.push_back(0);
+            wg.push_back(std::complex<double>(w[2 * n], w[2 * n + 1]) * kw[ik] * s);
+        }
+    return diagonal_jobs(c, en, je, jk, wg, tg, 1, kx, ky, nk, density_matrix, true);
+}
+
+int negf_ldos_batch(negf_ctx *c, const double *E, int nE, const double *kx, const double *ky,
+                    const double *kw, int nk, double *ldos_matrix) {
+    int st = check_ctx(c);
+    if (st) return st;
+    if (nE < 0 || nk < 1 || !kx || !ky || !kw || (nE > 0 && (!E || !ldos_matrix)))
+        return fail(-2, "bad arguments to negf_ldos_batch");
+    if (nE == 0) return 0;
+    const double delta = c->Lz / c->Nz, s = 2 * delta * delta;
+    std::vector<std::complex<double>> en(nE), wg;
+    std::vector<int> je, jk, tg;
+    for (int i = 0; i < nE; ++i) en[i] = E[i];
+    for (int ik = 0; ik < nk; ++ik)
+        for (int i = 0; i < nE; ++i) {
+            je.push_back(i);
+            jk.push_back(ik);
+            tg.push_back(i);
+            wg.push_back(-kw[ik] / PI * s);   // src/negf.f90:230-231
+        }
+    return diagonal_jobs(c, en, je, jk, wg, tg, nE, kx, ky, nk, ldos_matrix, false);
+}
+
+int negf_inverse_batch(int N, int count, const double *A, double *inv) {
+    if (N < 1 || count < 0 || (count > 0 && (!A || !inv))) return fail(-2, "bad arguments to negf_inverse_batch");
+    if (count == 0) return 0;
+    const int Np = round_up(N, 16);
+    if (Np > NEGF_MAXNP) return fail(-5, "N too large (N <= 1024)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-7, "no GPU visible");
+    Temps tmp;
+    cd *dA, *dW, *dI;
+    int *dS;
+    int st;
+    const size_t nn = (size_t)N * N * count;
+    if ((st = tmp.upload(&dA, reinterpret_cast<const cd *>(A), nn, nullptr)) ||
+        (st = tmp.alloc(&dW, (size_t)Np * Np * count)) || (st = tmp.alloc(&dI, nn)) ||
+        (st = tmp.alloc(&dS, count)))
+        return st;
+    if (threads_for(Np) <= 512)
+        hipLaunchKernelGGL(k_inverse_batch<512>, dim3(count), dim3(threads_for(Np)), 0, nullptr, N, Np, dA,
+                           (const cd *)nullptr, dW, dI, dS);
+    else
+        hipLaunchKernelGGL(k_inverse_batch<1024>, dim3(count), dim3(threads_for(Np)), 0, nullptr, N, Np, dA,
+                           (const cd *)nullptr, dW, dI, dS);
+    HIPCHK(hipGetLastError());
+    std::vector<int> hs(count);
+    HIPCHK(hipMemcpy(inv, dI, nn * sizeof(cd), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hs.data(), dS, count * sizeof(int), hipMemcpyDeviceToHost));
+    for (int b = 0; b < count; ++b)
+        if (hs[b]) return b + 1;
+    return 0;
+}
+
+int negf_matrix_product_batch(int N, int count, const double *A, const double *B, double *C) {
+    if (N < 1 || count < 0 || (count > 0 && (!A || !B || !C))) return fail(-2, "bad arguments to negf_matrix_product_batch");
+    if (count == 0) return 0;
+    const int Np = round_up(N, 16);
+    if (Np > NEGF_MAXNP) return fail(-5, "N too large (N <= 1024)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-7, "no GPU visible");
+    Temps tmp;
+    cd *dA, *dB, *dC, *dW;
+    int st;
+    const size_t nn = (size_t)N * N * count;
+    if ((st = tmp.upload(&dA, reinterpret_cast<const cd *>(A), nn, nullptr)) ||
+        (st = tmp.upload(&dB, reinterpret_cast<const cd *>(B), nn, nullptr)) ||
+        (st = tmp.alloc(&dC, nn)) || (st = tmp.alloc(&dW, (size_t)2 * Np * Np * count)))
+        return st;
+    const long s = (long)N * N;
+    if (threads_for(Np) <= 512)
+        hipLaunchKernelGGL(k_gemm_batch<512>, dim3(count), dim3(threads_for(Np)), 0, nullptr, N, Np, dA, s,
+                           dB, s, dC, s, 1.0, 0, dW);
+    else
+        hipLaunchKernelGGL(k_gemm_batch<1024>, dim3(count), dim3(threads_for(Np)), 0, nullptr, N, Np, dA, s,
+                           dB, s, dC, s, 1.0, 0, dW);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(C, dC, nn * sizeof(cd), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Generic tri-diagonal solver with identity off-diagonal blocks, host-orchestrated over
+// the batched primitives (one launch per slice step), following
+// src/math_kernel.f90:80-162 operation by operation.
+int negf_rgf_tri_solve(int N, int N_z, const double *G_inv_D, int mode, double *diag,
+                       double *first, double *last) {
+    if (N < 1 || N_z < 2 || !G_inv_D) return fail(-2, "bad arguments to negf_rgf_tri_solve");
+    if ((mode & 1) && !diag) return fail(-2, "mode D needs diag");
+    if ((mode & 2) && !first) return fail(-2, "mode F needs first");
+    if ((mode & 4) && !last) return fail(-2, "mode L needs last");
+    const int Np = round_up(N, 16);
+    if (Np > NEGF_MAXNP) return fail(-5, "N too large (N <= 1024)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-7, "no GPU visible");
+    Temps tmp;
+    const size_t NN = (size_t)N * N;
+    const int th = threads_for(Np);
+    cd *dD, *gr, *gl, *dOut, *dW, *dM;
+    int *dS;
+    int st;
+    if ((st = tmp.upload(&dD, reinterpret_cast<const cd *>(G_inv_D), NN * N_z, nullptr)) ||
+        (st = tmp.alloc(&gr, NN * (N_z - 1))) || (st = tmp.alloc(&gl, NN * (N_z - 1))) ||
+        (st = tmp.alloc(&dOut, NN * N_z)) || (st = tmp.alloc(&dW, (size_t)2 * Np * Np)) ||
+        (st = tmp.alloc(&dM, NN)) || (st = tmp.alloc(&dS, 1)))
+        return st;
+    int status = 0;
+    std::vector<int> hs(1);
+    auto inv = [&](const cd *A, const cd *Sub, cd *o, int slice) -> int {
+        if (th <= 512) hipLaunchKernelGGL(k_inverse_batch<512>, dim3(1), dim3(th), 0, nullptr, N, Np, A, Sub, dW, o, dS);
+        else hipLaunchKernelGGL(k_inverse_batch<1024>, dim3(1), dim3(th), 0, nullptr, N, Np, A, Sub, dW, o, dS);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(hs.data(), dS, sizeof(int), hipMemcpyDeviceToHost));
+        if (hs[0] && !status) status = slice;
+        return 0;
+    };
+    auto gemm = [&](const cd *A, const cd *B, cd *C, double alpha, int addI) -> int {
+        if (th <= 512)
+            hipLaunchKernelGGL(k_gemm_batch<512>, dim3(1), dim3(th), 0, nullptr, N, Np, A, 0L, B, 0L, C, 0L,
+                               alpha, addI, dW);
+        else
+            hipLaunchKernelGGL(k_gemm_batch<1024>, dim3(1), dim3(th), 0, nullptr, N, Np, A, 0L, B, 0L, C, 0L,
+                               alpha, addI, dW);
+        HIPCHK(hipGetLastError());
+        return 0;
+    };
+    const bool D = mode & 1, F = mode & 2, L = mode & 4;
+    if (D || F) {
+        if ((st = inv(dD + NN * (N_z - 1), nullptr, gr + NN * (N_z - 2), N_z))) return st;
+        for (int k = 2; k <= N_z - 1; ++k)
+            if ((st = inv(dD + NN * (N_z - k), gr + NN * (N_z - k), gr + NN * (N_z - k - 1), N_z - k + 1)))
+                return st;
+    }
+    if (D) {
+        if ((st = inv(dD, gr, dOut, 1))) return st;
+        for (int k = 2; k <= N_z; ++k) {
+            if ((st = gemm(dOut + NN * (k - 2), gr + NN * (k - 2), dM, 1.0, 1))) return st;
+            if ((st = gemm(gr + NN * (k - 2), dM, dOut + NN * (k - 1), 1.0, 0))) return st;
+        }
+        HIPCHK(hipMemcpy(diag, dOut, NN * N_z * sizeof(cd), hipMemcpyDeviceToHost));
+    }
+    if (F) {
+        if ((st = inv(dD, gr, dOut, 1))) return st;
+        for (int k = 2; k <= N_z; ++k)
+            if ((st = gemm(gr + NN * (k - 2), dOut + NN * (k - 2), dOut + NN * (k - 1), -1.0, 0))) return st;
+        HIPCHK(hipMemcpy(first, dOut, NN * N_z * sizeof(cd), hipMemcpyDeviceToHost));
+    }
+    if (L) {
+        if ((st = inv(dD, nullptr, gl, 1))) return st;
+        for (int k = 2; k <= N_z - 1; ++k)
+            if ((st = inv(dD + NN * (k - 1), gl + NN * (k - 2), gl + NN * (k - 1), k))) return st;
+        if ((st = inv(dD + NN * (N_z - 1), gl + NN * (N_z - 2), dOut + NN * (N_z - 1), N_z))) return st;
+        for (int k = 1; k <= N_z - 1; ++k)
+            if ((st = gemm(gl + NN * (N_z - k - 1), dOut + NN * (N_z - k), dOut + NN * (N_z - k - 1), -1.0, 0)))
+                return st;
+        HIPCHK(hipMemcpy(last, dOut, NN * N_z * sizeof(cd), hipMemcpyDeviceToHost));
+    }
+    HIPCHK(hipDeviceSynchronize());
+    return status;
+}
+
+static void twiddles(int n, double sign, std::vector<cd> &t) {
+    t.resize(n);
+    for (int j = 0; j < n; ++j) {
+        // angle 2 pi j / n, evaluated with the index reduced so cos/sin stay accurate
+        const double ang = 2.0 * PI * (double)j / (double)n;
+        t[j] = make_double2(std::cos(ang), sign * std::sin(ang));
+    }
+}
+
+int negf_pw_to_real(int N, const int *nx_grid, const int *ny_grid, int nmat, const double *mats,
+                    int N_x, int N_y, double *out) {
+    if (N < 1 || nmat < 0 || N_x < 1 || N_y < 1 || !nx_grid || !ny_grid || (nmat > 0 && (!mats || !out)))
+        return fail(-2, "bad arguments to negf_pw_to_real");
+    if (nmat == 0) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-7, "no GPU visible");
+    std::vector<cd> twx, twy;
+    twiddles(N_x, +1.0, twx);
+    twiddles(N_y, +1.0, twy);
+    Temps tmp;
+    int *dnx, *dny;
+    cd *dM, *dtx, *dty, *dS, *dO;
+    int st;
+    const size_t g = (size_t)N_x * N_y;
+    if ((st = tmp.upload(&dnx, nx_grid, N, nullptr)) || (st = tmp.upload(&dny, ny_grid, N, nullptr)) ||
+        (st = tmp.upload(&dM, reinterpret_cast<const cd *>(mats), (size_t)N * N * nmat, nullptr)) ||
+        (st = tmp.upload(&dtx, twx.data(), N_x, nullptr)) || (st = tmp.upload(&dty, twy.data(), N_y, nullptr)) ||
+        (st = tmp.alloc(&dS, 2 * g * nmat)) || (st = tmp.alloc(&dO, g * nmat)))
+        return st;
+    hipLaunchKernelGGL(k_pw_to_real, dim3(nmat), dim3(256), 0, nullptr, N, dnx, dny, dM, N_x, N_y, dtx,
+                       dty, dS, dO);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out, dO, g * nmat * sizeof(cd), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int negf_vhat_from_real(int N_x, int N_y, int N_z, const double *V_real, int NGX, int NGY,
+                        int full_window, double *vrec) {
+    if (N_x < 1 || N_y < 1 || N_z < 1 || !V_real || !vrec || NGX < 0 || NGY < 0)
+        return fail(-2, "bad arguments to negf_vhat_from_real");
+    // src/grid.f90:33-37 and the window bound of :58-60
+    if (2 * NGX > N_x || 2 * NGY > N_y) return fail(-3, "size of V_reciprocal minus one larger than V_real");
+    if (NGX > N_x / 2 || NGY + 1 + N_y / 2 > N_y) return fail(-3, "V_reciprocal window does not fit the real-space grid");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-7, "no GPU visible");
+    std::vector<cd> twx, twy;
+    twiddles(N_x, -1.0, twx);
+    twiddles(N_y, -1.0, twy);
+    Temps tmp;
+    double *dV;
+    cd *dtx, *dty, *dS, *dO;
+    int st;
+    const size_t nv = (size_t)(2 * NGX + 1) * (2 * NGY + 1) * N_z;
+    if ((st = tmp.upload(&dV, V_real, (size_t)N_x * N_y * N_z, nullptr)) ||
+        (st = tmp.upload(&dtx, twx.data(), N_x, nullptr)) || (st = tmp.upload(&dty, twy.data(), N_y, nullptr)) ||
+        (st = tmp.alloc(&dS, (size_t)(NGX + 1) * N_y * N_z)) || (st = tmp.alloc(&dO, nv)))
+        return st;
+    const int hx = full_window ? NGX : NGX / 2, hy = full_window ? NGY : NGY / 2;
+    hipLaunchKernelGGL(k_vhat, dim3(N_z), dim3(256), 0, nullptr, N_x, N_y, dV, NGX, NGY, hx, hy, dtx,
+                       dty, dS, dO);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(vrec, dO, nv * sizeof(cd), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // extern "C"

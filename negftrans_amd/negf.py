@@ -1,0 +1,362 @@
+"""Python host side of the MI355X RGF energy loop, mirroring NEGFtrans's interfaces.
+
+Every numeric routine of the energy loop runs in libnegf_gpu.so (HIP, gfx950) through
+the C-ABI of include/negf_gpu.h; this module only prepares arguments the way the
+reference's host code does (basis, k mesh, energy grids, contour nodes, unit
+conversion) and returns results in the reference's array layouts (Fortran order,
+(N, N, N_z) blocks).  Citations are to the NEGFtrans Fortran sources.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, iptr
+
+# src/global.f90:24-28
+HARTREE = 27.211386245988
+K_B = 8.617333262e-5 / HARTREE
+A_0 = 0.529177210903
+PI = 3.141592653589793
+
+MODE_D, MODE_F, MODE_L = 1, 2, 4
+PREC_FP64, PREC_MIXED_FP32 = 0, 1
+FULL_WINDOW = 16
+
+
+# ------------------------------------------------------------------ parameters
+@dataclass
+class Atomic:
+    """t_parameters (src/global.f90:5-7) in atomic units."""
+    V_L: float = 0.0
+    V_R: float = 0.0
+    MU: float = 5.568 / HARTREE
+    ETA: float = 1e-5
+    TEMPERATURE: float = 20.0
+    LX: float = 0.0
+    LY: float = 0.0
+    LZ: float = 0.0
+    ENCUT: float = 100.0 / HARTREE
+    GAP: float = 6.0 * K_B * 20.0
+    VDS: float = 0.0
+    TRANSMISSION_GRID: tuple = (0.0, 0.0)
+    LDOS_ENERGY_GRID: tuple = (0.0, 0.0)
+
+    @classmethod
+    def from_input(cls, V_L=0.0, V_R=0.0, MU=5.568, ETA=1e-5, TEMPERATURE=20.0, LX=0.0, LY=0.0,
+                   LZ=0.0, ENCUT=100.0, GAP=6.0, VDS=0.0, TRANSMISSION_GRID=(0.0, 0.0, 0.0),
+                   LDOS_ENERGY_GRID=(0.0, 0.0, 0.0), **_ignored):
+        """INPUT namelist values (eV, Angstrom, K; ETA in Hartree) -> atomic units
+        (src/main.f90:182-198)."""
+        return cls(V_L=V_L / HARTREE, V_R=V_R / HARTREE, MU=MU / HARTREE, ETA=ETA,
+                   TEMPERATURE=TEMPERATURE, LX=LX / A_0, LY=LY / A_0, LZ=LZ / A_0,
+                   ENCUT=ENCUT / HARTREE, GAP=GAP * K_B * TEMPERATURE, VDS=VDS / HARTREE,
+                   TRANSMISSION_GRID=(TRANSMISSION_GRID[0] / HARTREE, TRANSMISSION_GRID[1] / HARTREE),
+                   LDOS_ENERGY_GRID=(LDOS_ENERGY_GRID[0] / HARTREE, LDOS_ENERGY_GRID[1] / HARTREE))
+
+
+def ngx_auto(NGX, NGY, LX_ang, LY_ang, ENCUT_eV):
+    """src/main.f90:94-100 (the NGX test compares against the LY bound, as there)."""
+    bx = math.ceil((LX_ang / A_0) / PI * math.sqrt(ENCUT_eV / HARTREE / 2) * 2)
+    by = math.ceil((LY_ang / A_0) / PI * math.sqrt(ENCUT_eV / HARTREE / 2) * 2)
+    if NGX <= by:
+        NGX = bx + 1
+    if NGY <= by:
+        NGY = by + 1
+    return NGX, NGY
+
+
+def planewave_basis(ENCUT, Lx, Ly):
+    """PlaneWaveBasis_construction (src/grid.f90:387-431): n_y outer, n_x inner."""
+    nxm = math.ceil(Lx / PI * math.sqrt(ENCUT / 2))
+    nym = math.ceil(Ly / PI * math.sqrt(ENCUT / 2))
+    j, i = np.meshgrid(np.arange(-nym, nym + 1), np.arange(-nxm, nxm + 1), indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    keep = ((2 * i * PI / Lx) ** 2 + (2 * j * PI / Ly) ** 2) / 2 <= ENCUT
+    return i[keep].astype(np.int32), j[keep].astype(np.int32)
+
+
+def kpoint_mesh(NKX, NKY, Lx, Ly):
+    """Kpoint_mesh_construction (src/grid.f90:433-448): x fastest."""
+    i = np.tile(np.arange(1, NKX + 1), NKY)
+    j = np.repeat(np.arange(1, NKY + 1), NKX)
+    kx = (i - (1 + NKX) / 2.0) / NKX * (2 * PI / Lx)
+    ky = (j - (1 + NKY) / 2.0) / NKY * (2 * PI / Ly)
+    w = np.full(NKX * NKY, 1.0 / (NKX * NKY))
+    return kx, ky, w
+
+
+def uniform_energies(e0, e1, n):
+    """src/main.f90:369-375 / src/negf.f90:203-204: E_i = e0 + (e1-e0)(i-1)/n, i=1..n."""
+    return np.array([e0 + (e1 - e0) * (i - 1) / n for i in range(1, n + 1)])
+
+
+def coefficient_simpson(i, N, a, b):
+    """src/math_kernel.f90:167-204 (1-based i)."""
+    if N % 2 == 1:
+        c = 1.0 / 3 if i in (1, N) else (4.0 / 3 if i % 2 == 0 else 2.0 / 3)
+    elif i == 1:
+        c = 1.0 / 3
+    elif i == N - 3:
+        c = 1.0 / 3 + 1.0 * 3 / 8
+    elif i in (N - 2, N - 1):
+        c = 3.0 * 3 / 8
+    elif i == N:
+        c = 1.0 * 3 / 8
+    else:
+        c = 4.0 / 3 if i % 2 == 0 else 2.0 / 3
+    return c * (b - a) / (N - 1)
+
+
+def value_simpson(i, N, a, b):
+    """src/math_kernel.f90:206-216."""
+    return (i - 1) * (b - a) / (N - 1) + a
+
+
+def fermi_func(energy, mu, temperature):
+    """src/math_kernel.f90:218-223."""
+    return 1.0 / (math.exp((energy - mu) / (K_B * temperature)) + 1)
+
+
+def n_line(atomic: Atomic, N_line_per_eV: int) -> int:
+    """src/main.f90:261: IDNINT(N_line_per_eV * 2 GAP[eV])."""
+    x = N_line_per_eV * ((2 * atomic.GAP) * HARTREE)
+    return int(math.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
+
+
+def density_contour(atomic: Atomic, N_circle: int, N_line: int, min_V: float):
+    """Contour nodes z_n (without eta) and weights w_n of Equilibrium_Density
+    (src/negf.f90:36-65, 86-111); Density_Matrix += w_k w_n G_kk(z_n + i eta)."""
+    circle_L = min_V - 1.0 / HARTREE
+    circle_R = atomic.MU - atomic.GAP
+    line_L, line_R = atomic.MU - atomic.GAP, atomic.MU + atomic.GAP
+    E_c, E_R = (circle_R + circle_L) / 2, (circle_R - circle_L) / 2
+    z = np.empty(N_circle + N_line, dtype=np.complex128)
+    w = np.empty_like(z)
+    for i in range(1, N_circle + N_line + 1):
+        if i <= N_circle:
+            th = value_simpson(i, N_circle, 0.0, PI)
+            e = np.exp(1j * th)
+            z[i - 1] = E_c + E_R * e
+            w[i - 1] = 2.0 / PI * coefficient_simpson(i, N_circle, 0.0, PI) * 1j * E_R * e
+        else:
+            E = value_simpson(i - N_circle, N_line, line_L, line_R)
+            z[i - 1] = E
+            w[i - 1] = -(2.0 / PI * coefficient_simpson(i - N_circle, N_line, line_L, line_R)
+                         * fermi_func(E, atomic.MU, atomic.TEMPERATURE))
+    return z, w
+
+
+# ------------------------------------------------------------------ GPU entry points
+def device_count():
+    return _lib.load().negf_device_count()
+
+
+def local_potential_real_to_planewave(V_real, NGX, NGY, full_window=False):
+    """V_reciprocal_all(-NGX:NGX, -NGY:NGY, N_z) from V_real(N_x, N_y, N_z) (Hartree):
+    LocalPotential_RealToPlanewave per slice + the window copy (src/grid.f90:12-69,
+    src/main.f90:238-247).  Returned as a Fortran-ordered (2NGX+1, 2NGY+1, N_z) array."""
+    V = np.asfortranarray(np.asarray(V_real, dtype=np.float64))
+    N_x, N_y, N_z = V.shape
+    out = np.zeros((2 * NGX + 1, 2 * NGY + 1, N_z), dtype=np.complex128, order="F")
+    lib = _lib.load()
+    check(lib.negf_vhat_from_real(N_x, N_y, N_z, dptr(V), NGX, NGY, int(bool(full_window)),
+                                  dptr(out)), "negf_vhat_from_real")
+    return out
+
+
+def inverse(M):
+    """Batched drop-in for inverse (src/math_kernel.f90:8-34). M: (N, N) or (N, N, B)."""
+    A = np.asfortranarray(np.asarray(M, dtype=np.complex128))
+    squeeze = A.ndim == 2
+    if squeeze:
+        A = A[:, :, None]
+    N, _, B = A.shape
+    out = np.zeros_like(A, order="F")
+    check(_lib.load().negf_inverse_batch(N, B, dptr(A), dptr(out)), "negf_inverse_batch")
+    return out[:, :, 0] if squeeze else out
+
+
+def matrix_product(A, B):
+    """Batched drop-in for matrix_product (src/math_kernel.f90:36-42)."""
+    A = np.asfortranarray(np.asarray(A, dtype=np.complex128))
+    Bm = np.asfortranarray(np.asarray(B, dtype=np.complex128))
+    squeeze = A.ndim == 2
+    if squeeze:
+        A, Bm = A[:, :, None], Bm[:, :, None]
+    N, _, nb = A.shape
+    C = np.zeros_like(A, order="F")
+    check(_lib.load().negf_matrix_product_batch(N, nb, dptr(A), dptr(Bm), dptr(C)),
+          "negf_matrix_product_batch")
+    return C[:, :, 0] if squeeze else C
+
+
+def greensfunction_tri_solver(G_inv_D, DFL):
+    """GreensFunction_tri_solver (src/math_kernel.f90:44-165).  G_inv_D: (N, N, N_z)
+    complex128; returns dict of (N, N, N_z) arrays 'diagonal', 'first_column',
+    'last_column' (zero where not requested) and the status code."""
+    D = np.asfortranarray(np.asarray(G_inv_D, dtype=np.complex128))
+    N, _, N_z = D.shape
+    mode = (MODE_D if any(c in DFL for c in "Dd") else 0) | \
+           (MODE_F if any(c in DFL for c in "Ff") else 0) | \
+           (MODE_L if any(c in DFL for c in "Ll") else 0)
+    outs = {k: np.zeros((N, N, N_z), dtype=np.complex128, order="F")
+            for k in ("diagonal", "first_column", "last_column")}
+    nul = ctypes.POINTER(ctypes.c_double)()
+    st = check(_lib.load().negf_rgf_tri_solve(
+        N, N_z, dptr(D), mode,
+        dptr(outs["diagonal"]) if mode & MODE_D else nul,
+        dptr(outs["first_column"]) if mode & MODE_F else nul,
+        dptr(outs["last_column"]) if mode & MODE_L else nul), "negf_rgf_tri_solve")
+    outs["status"] = st
+    return outs
+
+
+def greensfunction_planewave_to_real(M, nx, ny, N_x, N_y):
+    """Greensfunction_PlanewaveToReal (src/grid.f90:71-135) for one (N, N) block or a
+    stack (N, N, B); returns (N_x, N_y) or (N_x, N_y, B) complex."""
+    A = np.asfortranarray(np.asarray(M, dtype=np.complex128))
+    squeeze = A.ndim == 2
+    if squeeze:
+        A = A[:, :, None]
+    N, _, B = A.shape
+    nx = np.ascontiguousarray(nx, dtype=np.int32)
+    ny = np.ascontiguousarray(ny, dtype=np.int32)
+    out = np.zeros((N_x, N_y, B), dtype=np.complex128, order="F")
+    check(_lib.load().negf_pw_to_real(N, iptr(nx), iptr(ny), B, dptr(A), N_x, N_y, dptr(out)),
+          "negf_pw_to_real")
+    return out[:, :, 0] if squeeze else out
+
+
+class NegfDevice:
+    """One transport system resident on the GPU (basis + V_reciprocal_all + geometry).
+
+    Replaces the per-job setup of src/negf.f90 (Hamiltonian_construction and
+    E_minus_H_construction, rebuilt for every energy there) with blocks generated on
+    the fly inside the sweep kernels."""
+
+    def __init__(self, nx, ny, atomic: Atomic, vrec, NGX, NGY, flags=PREC_FP64):
+        self.nx = np.ascontiguousarray(nx, dtype=np.int32)
+        self.ny = np.ascontiguousarray(ny, dtype=np.int32)
+        self.atomic = atomic
+        self.vrec = np.asfortranarray(np.asarray(vrec, dtype=np.complex128))
+        self.NGX, self.NGY = NGX, NGY
+        self.N = len(self.nx)
+        self.N_z = self.vrec.shape[2]
+        assert self.vrec.shape[:2] == (2 * NGX + 1, 2 * NGY + 1)
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(self._lib.negf_ctx_create(ctypes.byref(h), self.N, self.N_z, iptr(self.nx),
+                                        iptr(self.ny), atomic.LX, atomic.LY, atomic.LZ, atomic.V_L,
+                                        atomic.V_R, atomic.ETA, dptr(self.vrec), NGX, NGY, flags),
+              "negf_ctx_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.negf_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def last_kernel_seconds(self):
+        return self._lib.negf_last_kernel_seconds(self._h)
+
+    @staticmethod
+    def _kp(kpoints):
+        if kpoints is None:
+            kpoints = (np.zeros(1), np.zeros(1), np.ones(1))
+        kx, ky, kw = (np.ascontiguousarray(a, dtype=np.float64) for a in kpoints)
+        return kx, ky, kw
+
+    def transmission(self, energies, kpoints=None, per_job=False):
+        """T(E_i) = sum_k w_k tau(E_i, k) for real energies (Hartree): the transmission
+        job loop (src/main.f90:384-398) over Transmission_Coefficient
+        (src/negf.f90:115-174), all jobs in one launch."""
+        E = np.ascontiguousarray(energies, dtype=np.float64)
+        kx, ky, kw = self._kp(kpoints)
+        tau = np.zeros(len(E))
+        jobs = np.zeros(len(E) * len(kx))
+        st = check(self._lib.negf_transmission_batch(self._h, dptr(E), len(E), dptr(kx), dptr(ky),
+                                                     dptr(kw), len(kx), dptr(tau), dptr(jobs)),
+                   "negf_transmission_batch")
+        self.last_status = st
+        return (tau, jobs.reshape(len(kx), len(E))) if per_job else tau
+
+    def density_matrix(self, z, w, kpoints=None, out=None):
+        """Density_Matrix(N, N, N_z) += sum_k sum_n w_k w_n G_kk(z_n + i eta)
+        (src/main.f90:272-288 over Equilibrium_Density, src/negf.f90:9-113)."""
+        zz = np.ascontiguousarray(z, dtype=np.complex128)
+        ww = np.ascontiguousarray(w, dtype=np.complex128)
+        kx, ky, kw = self._kp(kpoints)
+        if out is None:
+            out = np.zeros((self.N, self.N, self.N_z), dtype=np.complex128, order="F")
+        self.last_status = check(self._lib.negf_density_batch(
+            self._h, dptr(zz), dptr(ww), len(zz), dptr(kx), dptr(ky), dptr(kw), len(kx), dptr(out)),
+            "negf_density_batch")
+        return out
+
+    def ldos_matrix(self, energies, kpoints=None):
+        """LDOS_Matrix(N, N, N_z, nE) = -sum_k w_k/pi G_kk(E + i eta)
+        (Local_Density_Of_State, src/negf.f90:176-236)."""
+        E = np.ascontiguousarray(energies, dtype=np.float64)
+        kx, ky, kw = self._kp(kpoints)
+        out = np.zeros((self.N, self.N, self.N_z, len(E)), dtype=np.complex128, order="F")
+        self.last_status = check(self._lib.negf_ldos_batch(
+            self._h, dptr(E), len(E), dptr(kx), dptr(ky), dptr(kw), len(kx), dptr(out)),
+            "negf_ldos_batch")
+        return out
+
+
+# ------------------------------------------------------------------ output reductions
+def density_from_matrix(DM, nx, ny, N_x, N_y, LX_ang, LY_ang, LZ_ang):
+    """src/main.f90:306-350: plane-wave -> real space per slice on the GPU, then
+    Density = Im(.) * N_z / (LX LY LZ) [Angstrom^-3]; returns (N_x, N_y, N_z)."""
+    N_z = DM.shape[2]
+    out = greensfunction_planewave_to_real(DM, nx, ny, N_x, N_y)
+    return out.imag * (N_z / (LX_ang * LY_ang * LZ_ang))
+
+
+def ldos_size(v, n):
+    """src/tools.f90:6-16."""
+    return n if v == 0 else 1
+
+
+def ldos_reduce(LM, nx, ny, N_x, N_y, LX_ang, LY_ang, LZ_ang, LDOS_GRID):
+    """src/main.f90:481-525 + LDOS_grid_converter (src/tools.f90:18-40) for one energy:
+    transform every slice, keep (0) / integrate (-1) / select (m) per dimension,
+    rescale to Angstrom^-n eV^-1."""
+    N_z = LM.shape[2]
+    im = greensfunction_planewave_to_real(LM, nx, ny, N_x, N_y).imag   # (N_x, N_y, N_z)
+    base = N_z / (LX_ang * LY_ang * LZ_ang) / HARTREE
+    axes = []
+    for v, n, d in ((LDOS_GRID[0], N_x, LX_ang / N_x), (LDOS_GRID[1], N_y, LY_ang / N_y),
+                    (LDOS_GRID[2], N_z, LZ_ang / N_z)):
+        if v == 0:
+            axes.append(("keep", None, 1.0))
+        elif v == -1:
+            axes.append(("sum", None, d))
+        else:
+            axes.append(("pick", v, 1.0))
+    out = im * base
+    for ax, (kind, m, f) in enumerate(axes):
+        if kind == "sum":
+            out = out.sum(axis=ax, keepdims=True) * f
+        elif kind == "pick":
+            n = out.shape[ax]
+            sel = np.zeros(n)
+            if 1 <= m <= n:
+                sel[m - 1] = 1.0
+            shape = [1, 1, 1]
+            shape[ax] = n
+            out = (out * sel.reshape(shape)).sum(axis=ax, keepdims=True)
+    return out

@@ -1,0 +1,221 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Bar: relative 1e-10 for every FP64 output (the same tolerance is written in each
+assertion), on seeded inputs at sizes the oracle finishes in seconds, plus the
+full-size metric configuration (N = 317, N_z = 1000) on two energies.
+"""
+import numpy as np
+import pytest
+
+from oracle import negf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def ng():
+    from negftrans_amd import negf
+    if negf.device_count() < 1:
+        pytest.fail("no GPU visible to libnegf_gpu.so")
+    return negf
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300)
+
+
+def rand_c(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+# ------------------------------------------------------------------ a7, a8
+@pytest.mark.parametrize("N", [1, 5, 16, 17, 61, 100, 317])
+def test_inverse_batch(ng, N):
+    rng = np.random.default_rng(N)
+    A = rand_c(rng, N, N, 3)
+    A[:, :, 1] = A[:, :, 1] * 1e-3 + np.eye(N)[::-1]      # forces row interchanges
+    inv = ng.inverse(A)
+    for b in range(3):
+        ref = np.linalg.inv(A[:, :, b])
+        assert rel_err(inv[:, :, b], ref) < RTOL * max(1.0, np.linalg.cond(A[:, :, b]) / 1e3)
+        assert rel_err(inv[:, :, b], O.inverse(A[:, :, b])) < RTOL * max(1.0, np.linalg.cond(A[:, :, b]) / 1e3)
+
+
+@pytest.mark.parametrize("N", [3, 61, 317])
+def test_matrix_product_batch(ng, N):
+    rng = np.random.default_rng(7 + N)
+    A, B = rand_c(rng, N, N, 2), rand_c(rng, N, N, 2)
+    C = ng.matrix_product(A, B)
+    for b in range(2):
+        assert rel_err(C[:, :, b], O.matrix_product(A[:, :, b], B[:, :, b])) < RTOL
+
+
+def test_singular_block_reports_status(ng):
+    A = np.zeros((4, 4, 1), dtype=np.complex128)
+    import negftrans_amd._lib as L
+    out = np.zeros_like(A, order="F")
+    st = L.load().negf_inverse_batch(4, 1, L.dptr(np.asfortranarray(A)), L.dptr(out))
+    assert st == 1
+
+
+# ------------------------------------------------------------------ a9
+@pytest.mark.parametrize("N,N_z,DFL", [(13, 7, "DFL"), (5, 2, "DFL"), (61, 12, "DF"), (29, 9, "L")])
+def test_tri_solver(ng, N, N_z, DFL):
+    rng = np.random.default_rng(N * 31 + N_z)
+    D = rand_c(rng, N_z, N, N) * 0.5 + 3.0 * np.eye(N)
+    ref = O.tri_solver(D, DFL)
+    out = ng.greensfunction_tri_solver(np.moveaxis(D, 0, 2), DFL)
+    assert out["status"] == 0
+    for key in ("diagonal", "first_column", "last_column"):
+        r = np.moveaxis(ref[key], 0, 2)
+        if np.any(r != 0):
+            assert rel_err(out[key], r) < RTOL, key
+        else:
+            assert np.all(out[key] == 0), key
+
+
+# ------------------------------------------------------------------ small systems
+def small_system(N_z=8, seed=3, R2=6.5, NK=(1, 1), eta=1e-6, VL=0.0, VR=0.0, full=False):
+    L = 12.0
+    ENCUT = R2 * (2 * np.pi / L) ** 2 / 2
+    nx, ny = O.planewave_basis(ENCUT, L, L)
+    NGX = NGY = 2 * int(np.ceil(np.sqrt(R2))) + 1
+    N_x = N_y = 2 * NGX + 4
+    V = 0.02 * np.random.default_rng(seed).standard_normal((N_x, N_y, N_z)) + 0.01
+    Vrec = O.v_reciprocal_all(V, NGX, NGY, full_window=full)
+    a = O.Atomic(V_L=VL, V_R=VR, ETA=eta, LX=L, LY=L, LZ=N_z * 0.7, ENCUT=ENCUT)
+    kx, ky, kw = O.kpoint_mesh(NK[0], NK[1], L, L)
+    return O.System(a, nx, ny, Vrec, NGX, NGY, kx, ky, kw), V
+
+
+def device_for(ng, s):
+    a = s.atomic
+    at = ng.Atomic(V_L=a.V_L, V_R=a.V_R, ETA=a.ETA, LX=a.LX, LY=a.LY, LZ=a.LZ, ENCUT=a.ENCUT)
+    return ng.NegfDevice(s.nx, s.ny, at, s.Vrec, s.NGX, s.NGY)
+
+
+@pytest.mark.parametrize("NK,VL,VR,full", [((1, 1), 0.0, 0.0, False), ((2, 3), 0.0, 0.0, True),
+                                           ((1, 1), 0.02, -0.01, True)])
+def test_transmission_small(ng, NK, VL, VR, full):
+    s, _ = small_system(N_z=9, NK=NK, VL=VL, VR=VR, full=full)
+    E = np.linspace(0.05, 1.2, 17)
+    ref = O.transmission(s, E)
+    dev = device_for(ng, s)
+    got = dev.transmission(E, (s.kx, s.ky, s.kw))
+    assert dev.last_status == 0
+    np.testing.assert_allclose(got, ref, rtol=RTOL, atol=1e-12 * max(1.0, np.abs(ref).max()))
+
+
+def test_transmission_two_slices(ng):
+    s, _ = small_system(N_z=2, seed=9)
+    E = np.array([0.1, 0.5, 0.9])
+    ref = O.transmission(s, E)
+    got = device_for(ng, s).transmission(E)
+    np.testing.assert_allclose(got, ref, rtol=RTOL, atol=1e-13)
+
+
+# ------------------------------------------------------------------ workloads
+def oracle_system(wl, vrec, ngx, ngy):
+    a = O.atomic_from_input(**{k: v for k, v in wl.inp.items()
+                               if k in ("V_L", "V_R", "MU", "ETA", "TEMPERATURE", "LX", "LY", "LZ",
+                                        "ENCUT", "GAP", "VDS", "TRANSMISSION_GRID")})
+    nx, ny = O.planewave_basis(a.ENCUT, a.LX, a.LY)
+    return O.System(a, nx, ny, vrec, ngx, ngy)
+
+
+def test_vhat_c0(ng):
+    from negftrans_amd import workloads
+    wl = workloads.qpc_c0(n_energy=4)
+    NGX, NGY = wl.ngx()
+    V = wl.V_real_eV / O.HARTREE
+    for full in (False, True):
+        got = ng.local_potential_real_to_planewave(V, NGX, NGY, full_window=full)
+        ref = O.v_reciprocal_all(V, NGX, NGY, full_window=full)
+        assert rel_err(got, ref) < RTOL
+        assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max()
+
+
+def test_transmission_qpc_c0(ng):
+    """Config 0 geometry (N = 61, N_z = 200), 24 of its 400 energies."""
+    from negftrans_amd import workloads
+    wl = workloads.qpc_c0()
+    NGX, NGY = wl.ngx()
+    vrec = ng.local_potential_real_to_planewave(wl.V_real_eV / O.HARTREE, NGX, NGY)
+    s = oracle_system(wl, np.asarray(vrec), NGX, NGY)
+    assert s.N == 61 and s.N_z == 200
+    E = wl.energies_hartree()[::17][:24]
+    ref = O.transmission(s, E)
+    dev = ng.NegfDevice(s.nx, s.ny, wl.atomic(), vrec, NGX, NGY)
+    got = dev.transmission(E)
+    np.testing.assert_allclose(got, ref, rtol=RTOL, atol=1e-12)
+    assert np.all(got >= 0)
+
+
+def test_transmission_disorder_c1_full_size(ng):
+    """Metric configuration at full size (N = 317, N_z = 1000): two energies against the
+    oracle's O(N^2)-state restatement (same generators), plus size-independent bounds
+    on 64 energies: 0 <= T <= number of open channels."""
+    from negftrans_amd import workloads
+    wl = workloads.disorder_c1()
+    NGX, NGY = wl.ngx()
+    vrec = ng.local_potential_real_to_planewave(wl.V_real_eV / O.HARTREE, NGX, NGY)
+    a = wl.atomic()
+    dev = ng.NegfDevice(*wl.basis(), a, vrec, NGX, NGY)
+    assert dev.N == 317 and dev.N_z == 1000
+    Eall = wl.energies_hartree()
+    E = Eall[::64]
+    got = dev.transmission(E)
+    nx, ny = wl.basis()
+    kin = O.kinetic(nx, ny, a.LX, a.LY, 0.0, 0.0)
+    n_open = np.array([np.sum(2 * e - kin > 0) for e in E])
+    assert np.all(got >= -1e-12) and np.all(got <= n_open + 1e-6)
+    s = oracle_system(wl, np.asarray(vrec), NGX, NGY)
+    for i in (13, 50):
+        ref = O.transmission_tau(s, E[i], 0, faithful_solver=False)
+        assert got[i] == pytest.approx(ref, rel=RTOL, abs=1e-12)
+
+
+# ------------------------------------------------------------------ a12, a13, a14
+def test_density_small(ng):
+    s, V = small_system(N_z=5, seed=21, NK=(2, 1), eta=1e-4)
+    a = s.atomic
+    a.MU, a.TEMPERATURE = 0.6, 300.0
+    a.GAP = 6 * O.K_B * a.TEMPERATURE
+    minV = float(V.min())
+    ref = O.density_matrix(s, 7, 5, minV)
+    at = ng.Atomic(V_L=a.V_L, V_R=a.V_R, ETA=a.ETA, LX=a.LX, LY=a.LY, LZ=a.LZ, ENCUT=a.ENCUT,
+                   MU=a.MU, TEMPERATURE=a.TEMPERATURE, GAP=a.GAP)
+    z, w = ng.density_contour(at, 7, 5, minV)
+    dev = device_for(ng, s)
+    got = dev.density_matrix(z, w, (s.kx, s.ky, s.kw))
+    assert rel_err(got, np.moveaxis(ref, 0, 2)) < RTOL
+
+
+def test_ldos_small(ng):
+    s, V = small_system(N_z=6, seed=22, NK=(1, 2), eta=1e-3)
+    E = np.array([0.2, 0.7])
+    dev = device_for(ng, s)
+    got = dev.ldos_matrix(E, (s.kx, s.ky, s.kw))
+    for i, e in enumerate(E):
+        ref = O.ldos_matrix(s, e)
+        assert rel_err(got[:, :, :, i], np.moveaxis(ref, 0, 2)) < RTOL
+
+
+def test_pw_to_real(ng):
+    L = 6.0
+    nx, ny = O.planewave_basis(4.5 * (2 * np.pi / L) ** 2 / 2, L, L)
+    N = len(nx)
+    rng = np.random.default_rng(5)
+    M = rand_c(rng, N, N, 3)
+    got = ng.greensfunction_planewave_to_real(M, nx, ny, 6, 8)
+    for b in range(3):
+        ref = O.pw_to_real_literal(M[:, :, b], nx, ny, 6, 8)
+        assert rel_err(got[:, :, b], ref) < RTOL
+    nx, ny = O.planewave_basis(100.5 * (2 * np.pi / L) ** 2 / 2, L, L)
+    M = rand_c(rng, len(nx), len(nx), 2)
+    got = ng.greensfunction_planewave_to_real(M, nx, ny, 64, 64)
+    for b in range(2):
+        assert rel_err(got[:, :, b], O.pw_to_real(M[:, :, b], nx, ny, 64, 64)) < RTOL
