@@ -39,6 +39,14 @@ static int fail(int code, const std::string &msg) {
 static const double PI = 3.141592653589793;
 
 // ------------------------------------------------------------------ kernels
+// Only the open lead channels enter the trace: tau = sum gL_i gR_j |G_{N_z,1}(i,j)|^2 over
+// rows i with gL_i != 0 and columns j with gR_j != 0 (src/grid.f90:355-383; closed
+// channels have Q = 0 and sin(0) = 0).  The sweep therefore carries R = E_L^T g_{N_z} ...
+// g_m (the open rows of the running product, n_L x N) instead of the full N x N product:
+// R <- R g_m costs n_L N^2 per slice and is exact (no change of the arithmetic that
+// reaches the trace), while the inversion keeps its full N^3 cost.
+#define NEGF_RA 8   // rows of R accumulated per pass of the row product
+
 struct TransArgs {
     SysView sv;
     int njobs;
@@ -46,59 +54,114 @@ struct TransArgs {
     const int *kidx;      // per job k-point index
     const double *kinh;   // per k-point, Np each
     const cd *sigL, *sigR;      // per job, Np each
-    const double *gamL, *gamR;  // per job, Np each (0 for closed channels / padding)
-    cd *ws;               // per slot 3 Np^2
+    const int *open_off;  // per job offset into the open-channel lists
+    const int *nL, *nR;   // per job open-row / open-column counts
+    const int *Lrow, *Rcol;     // open row indices / open column indices (flat)
+    const double *gL, *gR;      // matching gamma values (flat)
+    int nLmax;            // max n_L over jobs (R buffers are nLmax x Np)
+    cd *ws;               // per slot 2 Np^2 + 2 nLmax Np
     double *tau;          // per job
     int *status;          // per job: first zero pivot (1-based slice) or 0
 };
 
-template <int MAXT>
+struct RowProdShared {
+    cd Rs[NEGF_RA * 64];
+};
+
+// Rn(a, c) = sum_k R(a, k) S(k, sig(c)), a < nL, c < Np (R, Rn: ld nLmax).
+__device__ __forceinline__ void row_product(cd *__restrict__ Rn, const cd *__restrict__ R, int nL,
+                                            int ldR, const cd *__restrict__ S, const int *sig,
+                                            int Np, RowProdShared &rs) {
+    const int c = threadIdx.x;
+    const cd *scol = S + (size_t)(c < Np ? sig[c] : 0) * Np;
+    for (int a0 = 0; a0 < nL; a0 += NEGF_RA) {
+        cd acc[NEGF_RA];
+#pragma unroll
+        for (int a = 0; a < NEGF_RA; ++a) acc[a] = cmk(0.0, 0.0);
+        for (int k0 = 0; k0 < Np; k0 += 64) {
+            for (int e = threadIdx.x; e < NEGF_RA * 64; e += blockDim.x) {
+                const int a = e % NEGF_RA, k = k0 + e / NEGF_RA;
+                rs.Rs[e] = (a0 + a < nL && k < Np) ? R[(a0 + a) + (size_t)k * ldR] : cmk(0.0, 0.0);
+            }
+            __syncthreads();
+            if (c < Np) {
+                const int kend = min(64, Np - k0);
+                for (int kk = 0; kk < kend; ++kk) {
+                    const cd sv = scol[k0 + kk];
+#pragma unroll
+                    for (int a = 0; a < NEGF_RA; ++a) acc[a] = cfma(rs.Rs[a + kk * NEGF_RA], sv, acc[a]);
+                }
+            }
+            __syncthreads();
+        }
+        if (c < Np) {
+#pragma unroll
+            for (int a = 0; a < NEGF_RA; ++a)
+                if (a0 + a < nL) Rn[(a0 + a) + (size_t)c * ldR] = acc[a];
+        }
+    }
+    __syncthreads();
+}
+
+template <int MAXT, int PB>
 __global__ __launch_bounds__(MAXT) void k_transmission(TransArgs a) {
-    __shared__ RgfShared s;
+    __shared__ GjeShared<PB> s;
+    __shared__ RowProdShared rs;
     const SysView &sv = a.sv;
-    const int Np = sv.Np;
+    const int Np = sv.Np, ldR = a.nLmax;
     const size_t NN = (size_t)Np * Np;
-    cd *buf0 = a.ws + (size_t)blockIdx.x * 3 * NN;
+    cd *base = a.ws + (size_t)blockIdx.x * (2 * NN + 2 * (size_t)ldR * Np + 2 * (size_t)PB * Np);
+    cd *scratch = base + 2 * NN + 2 * (size_t)ldR * Np;
     for (int job = blockIdx.x; job < a.njobs; job += gridDim.x) {
-        if (threadIdx.x == 0) s.flag = 0;
-        __syncthreads();
+        const int nL = a.nL[job], nR = a.nR[job], off = a.open_off[job];
+        if (nL == 0 || nR == 0) {             // no open channel: tau = 0 exactly
+            if (threadIdx.x == 0) {
+                a.tau[job] = 0.0;
+                a.status[job] = 0;
+            }
+            continue;
+        }
         JobView jv;
         jv.z = a.z[job];
         jv.kinh = a.kinh + (size_t)a.kidx[job] * Np;
         jv.sigL = a.sigL + (size_t)job * Np;
         jv.sigR = a.sigR + (size_t)job * Np;
-        const double *gL = a.gamL + (size_t)job * Np, *gR = a.gamR + (size_t)job * Np;
-        // S: latest generator (columns permuted by s.sigma), Q: running product, F: free
-        cd *S = buf0, *Q = buf0 + NN, *F = buf0 + 2 * NN;
+        cd *S = base, *M = base + NN;               // latest generator / block being inverted
+        cd *R = base + 2 * NN, *Rn = R + (size_t)ldR * Np;
         int bad = 0;
         double part = 0.0;
         for (int m = sv.Nz; m >= 1; --m) {
             // M = D_m - g_{m+1} (m = N_z: M = D_{N_z}), inverted in place
-            build_block(F, sv, jv, m - 1, m == sv.Nz ? nullptr : S, s.sigma);
+            build_block(M, sv, jv, m - 1, m == sv.Nz ? nullptr : S, s.sigma);
             __syncthreads();
-            if (gje_invert(F, Np, s) && !bad) bad = m;
+            if (gje_invert2<PB>(M, Np, s, scratch) && !bad) bad = m;
             if (m == sv.Nz) {
-                copy_cols(Q, F, s.sigma, Np);           // Q = g_{N_z}
+                // R = open rows of g_{N_z}
+                for (int e = threadIdx.x; e < nL * Np; e += blockDim.x) {
+                    const int ai = e % nL, c = e / nL;
+                    R[ai + (size_t)c * ldR] = M[a.Lrow[off + ai] + (size_t)s.sigma[c] * Np];
+                }
                 __syncthreads();
-                cd *t = S;
-                S = F;
-                F = t;
-                continue;
+            } else if (m >= 2) {
+                row_product(Rn, R, nL, ldR, M, s.sigma, Np, rs);    // R <- R g_m
+                cd *t = R;
+                R = Rn;
+                Rn = t;
+            } else {
+                // G_{N_z,1}(open rows, open cols) = +-R G_11(:, open cols); trace
+                for (int e = threadIdx.x; e < nL * nR; e += blockDim.x) {
+                    const int ai = e % nL, bi = e / nL;
+                    const cd *gc = M + (size_t)s.sigma[a.Rcol[off + bi]] * Np;
+                    cd acc = cmk(0.0, 0.0);
+                    for (int k = 0; k < Np; ++k) acc = cfma(R[ai + (size_t)k * ldR], gc[k], acc);
+                    part += a.gL[off + ai] * a.gR[off + bi] * (acc.x * acc.x + acc.y * acc.y);
+                }
             }
-            // m >= 2: Q <- Q g_m (into the buffer of g_{m+1}, no longer needed);
-            // m == 1: G_{N_z,1} = +-Q G_11, reduced straight into the trace.
-            const bool last = (m == 1);
-            cd *out = S;
-            gemm_nn(Q, F, s.sigma, Np, [&](int r, int c, cd v) {
-                if (last) part += gL[r] * gR[c] * (v.x * v.x + v.y * v.y);
-                else out[r + (size_t)c * Np] = v;
-            });
-            __syncthreads();
-            S = F;
-            F = Q;
-            Q = out;
+            cd *t = S;
+            S = M;
+            M = t;
         }
-        const double tot = block_sum(part, s);
+        const double tot = block_sum2(part, s.red);
         if (threadIdx.x == 0) {
             a.tau[job] = tot * sv.scale * sv.scale;
             a.status[job] = bad;
@@ -124,17 +187,16 @@ struct DiagArgs {
     int *status;
 };
 
-template <int MAXT>
+template <int MAXT, int PB>
 __global__ __launch_bounds__(MAXT) void k_diagonal(DiagArgs a) {
-    __shared__ RgfShared s;
+    __shared__ GjeShared<PB> s;
     const SysView &sv = a.sv;
     const int Np = sv.Np, N = sv.N, Nz = sv.Nz;
     const size_t NN = (size_t)Np * Np;
-    cd *G = a.ws + (size_t)blockIdx.x * (Nz + 2) * NN;   // G[m] = g_{m+1}, m = 1..Nz-1
+    cd *G = a.ws + (size_t)blockIdx.x * ((Nz + 2) * NN + 2 * (size_t)PB * Np);   // G[m] = g_{m+1}
+    cd *scratch = G + (Nz + 2) * NN;
     cd *X = G, *Y = G + (size_t)Nz * NN, *W = G + (size_t)(Nz + 1) * NN;
     for (int job = blockIdx.x; job < a.njobs; job += gridDim.x) {
-        if (threadIdx.x == 0) s.flag = 0;
-        __syncthreads();
         JobView jv;
         jv.z = a.z[job];
         jv.kinh = a.kinh + (size_t)a.kidx[job] * Np;
@@ -148,7 +210,7 @@ __global__ __launch_bounds__(MAXT) void k_diagonal(DiagArgs a) {
         for (int m = Nz; m >= 1; --m) {
             build_block(W, sv, jv, m - 1, m == Nz ? nullptr : G + (size_t)m * NN, nullptr);
             __syncthreads();
-            if (gje_invert(W, Np, s) && !bad) bad = m;
+            if (gje_invert2<PB>(W, Np, s, scratch) && !bad) bad = m;
             copy_cols(m >= 2 ? G + (size_t)(m - 1) * NN : X, W, s.sigma, Np);
             __syncthreads();
         }
@@ -191,16 +253,15 @@ __global__ __launch_bounds__(MAXT) void k_diagonal(DiagArgs a) {
 // and the building blocks of the generic tri-diagonal solver).
 // inv[b] = (A[b] - (Sub ? Sub[b] : 0))^{-1} for count matrices of N x N (ld N on both
 // sides); the work copy is padded to Np.
-template <int MAXT>
+template <int MAXT, int PB>
 __global__ __launch_bounds__(MAXT) void k_inverse_batch(int N, int Np, const cd *A, const cd *Sub,
                                                         cd *work, cd *inv, int *status) {
-    __shared__ RgfShared s;
+    __shared__ GjeShared<PB> s;
     const int b = blockIdx.x;
     const size_t NN = (size_t)Np * Np;
-    cd *Wk = work + b * NN;
+    cd *Wk = work + b * (NN + 2 * (size_t)PB * Np);
     const cd *Ab = A + (size_t)b * N * N;
     const cd *Sb = Sub ? Sub + (size_t)b * N * N : nullptr;
-    if (threadIdx.x == 0) s.flag = 0;
     for (int col = 0; col < Np; ++col)
         for (int r = threadIdx.x; r < Np; r += blockDim.x) {
             cd v = cmk(r == col ? 1.0 : 0.0, 0.0);
@@ -211,7 +272,7 @@ __global__ __launch_bounds__(MAXT) void k_inverse_batch(int N, int Np, const cd 
             Wk[r + (size_t)col * Np] = v;
         }
     __syncthreads();
-    const int zp = gje_invert(Wk, Np, s);
+    const int zp = gje_invert2<PB>(Wk, Np, s, Wk + NN);
     cd *ob = inv + (size_t)b * N * N;
     for (int col = 0; col < N; ++col) {
         const cd *sc = Wk + (size_t)s.sigma[col] * Np;
@@ -450,6 +511,14 @@ struct Temps {
 
 int threads_for(int Np) { return std::min(1024, round_up(Np, 64)); }
 
+// Padded block size: a multiple of the Gauss-Jordan panel width (32 up to 512, 16 above,
+// where one thread per row needs the 1024-thread workgroup and its smaller register file).
+int padded_size(int N) {
+    const int p32 = round_up(N, 32);
+    return p32 <= 512 ? p32 : round_up(N, 16);
+}
+int panel_width(int Np) { return Np <= 512 ? 32 : 16; }
+
 // Per-job host tables: kinetic/2 per k-point, lead terms and trace weights per job.
 struct JobTables {
     std::vector<cd> z, sigL, sigR;
@@ -561,7 +630,7 @@ int negf_ctx_create(negf_ctx **out, int N, int N_z, const int *nx_grid, const in
     if (N < 1 || N_z < 2) return fail(-2, "need N >= 1 and N_z >= 2");
     if (!nx_grid || !ny_grid || !vrec) return fail(-3, "null array argument");
     if (NGX < 0 || NGY < 0) return fail(-4, "NGX/NGY must be >= 0");
-    const int Np = round_up(N, 16);
+    const int Np = padded_size(N);
     if (Np > NEGF_MAXNP) return fail(-5, "N too large for one workgroup (N <= 1024)");
     for (int i = 0; i < N; ++i) {
         for (int j = 0; j < N; ++j) {
@@ -644,12 +713,38 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
     }
     JobTables t;
     job_tables(c, energies, realE, kx, ky, nk, je, jk, true, t);
+    // open-channel lists per job
+    std::vector<int> off(nj), nL(nj), nR(nj), Lrow, Rcol;
+    std::vector<double> gLv, gRv;
+    int nLmax = 1;
+    for (int j = 0; j < nj; ++j) {
+        off[j] = (int)Lrow.size();
+        int l = 0, r = 0;
+        std::vector<int> rc;
+        std::vector<double> rg;
+        for (int i = 0; i < c->N; ++i) {
+            const double gl = t.gamL[(size_t)j * c->Np + i], gr = t.gamR[(size_t)j * c->Np + i];
+            if (gl != 0.0) { Lrow.push_back(i); gLv.push_back(gl); ++l; }
+            if (gr != 0.0) { rc.push_back(i); rg.push_back(gr); ++r; }
+        }
+        // Lrow/Rcol share one offset: pad the shorter list so both start at off[j]
+        const int n = std::max(l, r);
+        for (int q = l; q < n; ++q) { Lrow.push_back(0); gLv.push_back(0.0); }
+        for (int q = 0; q < n; ++q) {
+            Rcol.push_back(q < r ? rc[q] : 0);
+            gRv.push_back(q < r ? rg[q] : 0.0);
+        }
+        nL[j] = l;
+        nR[j] = r;
+        nLmax = std::max(nLmax, l);
+    }
+    if (Lrow.empty()) { Lrow.push_back(0); Rcol.push_back(0); gLv.push_back(0); gRv.push_back(0); }
     Temps tmp;
     TransArgs a;
     a.sv = sysview(c);
     a.njobs = nj;
     cd *d_z, *d_sl, *d_sr;
-    int *d_k, *d_st;
+    int *d_k, *d_st, *d_off, *d_nL, *d_nR, *d_Lr, *d_Rc;
     double *d_kin, *d_gl, *d_gr, *d_tau;
     const size_t njp = (size_t)nj * c->Np;
     if ((st = tmp.upload(&d_z, t.z.data(), nj, c->stream)) ||
@@ -657,13 +752,19 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
         (st = tmp.upload(&d_kin, t.kinh.data(), (size_t)nk * c->Np, c->stream)) ||
         (st = tmp.upload(&d_sl, t.sigL.data(), njp, c->stream)) ||
         (st = tmp.upload(&d_sr, t.sigR.data(), njp, c->stream)) ||
-        (st = tmp.upload(&d_gl, t.gamL.data(), njp, c->stream)) ||
-        (st = tmp.upload(&d_gr, t.gamR.data(), njp, c->stream)) ||
+        (st = tmp.upload(&d_off, off.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_nL, nL.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_nR, nR.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_Lr, Lrow.data(), Lrow.size(), c->stream)) ||
+        (st = tmp.upload(&d_Rc, Rcol.data(), Rcol.size(), c->stream)) ||
+        (st = tmp.upload(&d_gl, gLv.data(), gLv.size(), c->stream)) ||
+        (st = tmp.upload(&d_gr, gRv.data(), gRv.size(), c->stream)) ||
         (st = tmp.alloc(&d_tau, nj)) || (st = tmp.alloc(&d_st, nj)))
         return st;
     const int threads = threads_for(c->Np);
-    const size_t per_slot = (size_t)3 * c->Np * c->Np * sizeof(cd);
-    const void *kfn = threads <= 512 ? (const void *)k_transmission<512> : (const void *)k_transmission<1024>;
+    const size_t per_slot =
+        ((size_t)2 * c->Np * c->Np + (size_t)2 * nLmax * c->Np + (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
+    const void *kfn = threads <= 512 ? (const void *)k_transmission<512, 32> : (const void *)k_transmission<1024, 16>;
     const int slots = slots_for(c, kfn, threads, nj, per_slot);
     if ((st = ensure_ws(c, per_slot * slots))) return st;
     a.z = d_z;
@@ -671,14 +772,20 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
     a.kinh = d_kin;
     a.sigL = d_sl;
     a.sigR = d_sr;
-    a.gamL = d_gl;
-    a.gamR = d_gr;
+    a.open_off = d_off;
+    a.nL = d_nL;
+    a.nR = d_nR;
+    a.Lrow = d_Lr;
+    a.Rcol = d_Rc;
+    a.gL = d_gl;
+    a.gR = d_gr;
+    a.nLmax = nLmax;
     a.ws = (cd *)c->ws;
     a.tau = d_tau;
     a.status = d_st;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    if (threads <= 512) hipLaunchKernelGGL(k_transmission<512>, dim3(slots), dim3(threads), 0, c->stream, a);
-    else hipLaunchKernelGGL(k_transmission<1024>, dim3(slots), dim3(threads), 0, c->stream, a);
+    if (threads <= 512) hipLaunchKernelGGL((k_transmission<512, 32>), dim3(slots), dim3(threads), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_transmission<1024, 16>), dim3(slots), dim3(threads), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     std::vector<double> h_tau(nj);
@@ -736,8 +843,9 @@ static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &e
         HIPCHK(hipMemsetAsync(d_out, 0, nout * sizeof(cd), c->stream));
     }
     const int threads = threads_for(c->Np);
-    const size_t per_slot = (size_t)(c->Nz + 2) * c->Np * c->Np * sizeof(cd);
-    const void *kfn = threads <= 512 ? (const void *)k_diagonal<512> : (const void *)k_diagonal<1024>;
+    const size_t per_slot =
+        ((size_t)(c->Nz + 2) * c->Np * c->Np + (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
+    const void *kfn = threads <= 512 ? (const void *)k_diagonal<512, 32> : (const void *)k_diagonal<1024, 16>;
     const int slots = slots_for(c, kfn, threads, nj, per_slot);
     if ((st = ensure_ws(c, per_slot * slots))) return st;
     a.z = d_z;
@@ -751,8 +859,8 @@ static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &e
     a.out = d_out;
     a.status = d_st;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    if (threads <= 512) hipLaunchKernelGGL(k_diagonal<512>, dim3(slots), dim3(threads), 0, c->stream, a);
-    else hipLaunchKernelGGL(k_diagonal<1024>, dim3(slots), dim3(threads), 0, c->stream, a);
+    if (threads <= 512) hipLaunchKernelGGL((k_diagonal<512, 32>), dim3(slots), dim3(threads), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_diagonal<1024, 16>), dim3(slots), dim3(threads), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     std::vector<int> h_st(nj);
@@ -949,7 +1057,7 @@ int negf_ldos_batch(negf_ctx *c, const double *E, int nE, const double *kx, cons
 int negf_inverse_batch(int N, int count, const double *A, double *inv) {
     if (N < 1 || count < 0 || (count > 0 && (!A || !inv))) return fail(-2, "bad arguments to negf_inverse_batch");
     if (count == 0) return 0;
-    const int Np = round_up(N, 16);
+    const int Np = padded_size(N);
     if (Np > NEGF_MAXNP) return fail(-5, "N too large (N <= 1024)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-7, "no GPU visible");
@@ -959,14 +1067,14 @@ int negf_inverse_batch(int N, int count, const double *A, double *inv) {
     int st;
     const size_t nn = (size_t)N * N * count;
     if ((st = tmp.upload(&dA, reinterpret_cast<const cd *>(A), nn, nullptr)) ||
-        (st = tmp.alloc(&dW, (size_t)Np * Np * count)) || (st = tmp.alloc(&dI, nn)) ||
+        (st = tmp.alloc(&dW, ((size_t)Np * Np + 2 * (size_t)panel_width(Np) * Np) * count)) || (st = tmp.alloc(&dI, nn)) ||
         (st = tmp.alloc(&dS, count)))
         return st;
     if (threads_for(Np) <= 512)
-        hipLaunchKernelGGL(k_inverse_batch<512>, dim3(count), dim3(threads_for(Np)), 0, nullptr, N, Np, dA,
+        hipLaunchKernelGGL((k_inverse_batch<512, 32>), dim3(count), dim3(threads_for(Np)), 0, nullptr, N, Np, dA,
                            (const cd *)nullptr, dW, dI, dS);
     else
-        hipLaunchKernelGGL(k_inverse_batch<1024>, dim3(count), dim3(threads_for(Np)), 0, nullptr, N, Np, dA,
+        hipLaunchKernelGGL((k_inverse_batch<1024, 16>), dim3(count), dim3(threads_for(Np)), 0, nullptr, N, Np, dA,
                            (const cd *)nullptr, dW, dI, dS);
     HIPCHK(hipGetLastError());
     std::vector<int> hs(count);
@@ -980,7 +1088,7 @@ int negf_inverse_batch(int N, int count, const double *A, double *inv) {
 int negf_matrix_product_batch(int N, int count, const double *A, const double *B, double *C) {
     if (N < 1 || count < 0 || (count > 0 && (!A || !B || !C))) return fail(-2, "bad arguments to negf_matrix_product_batch");
     if (count == 0) return 0;
-    const int Np = round_up(N, 16);
+    const int Np = padded_size(N);
     if (Np > NEGF_MAXNP) return fail(-5, "N too large (N <= 1024)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-7, "no GPU visible");
@@ -1013,7 +1121,7 @@ int negf_rgf_tri_solve(int N, int N_z, const double *G_inv_D, int mode, double *
     if ((mode & 1) && !diag) return fail(-2, "mode D needs diag");
     if ((mode & 2) && !first) return fail(-2, "mode F needs first");
     if ((mode & 4) && !last) return fail(-2, "mode L needs last");
-    const int Np = round_up(N, 16);
+    const int Np = padded_size(N);
     if (Np > NEGF_MAXNP) return fail(-5, "N too large (N <= 1024)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-7, "no GPU visible");
@@ -1025,14 +1133,14 @@ int negf_rgf_tri_solve(int N, int N_z, const double *G_inv_D, int mode, double *
     int st;
     if ((st = tmp.upload(&dD, reinterpret_cast<const cd *>(G_inv_D), NN * N_z, nullptr)) ||
         (st = tmp.alloc(&gr, NN * (N_z - 1))) || (st = tmp.alloc(&gl, NN * (N_z - 1))) ||
-        (st = tmp.alloc(&dOut, NN * N_z)) || (st = tmp.alloc(&dW, (size_t)2 * Np * Np)) ||
+        (st = tmp.alloc(&dOut, NN * N_z)) || (st = tmp.alloc(&dW, (size_t)2 * Np * Np + 2 * (size_t)panel_width(Np) * Np)) ||
         (st = tmp.alloc(&dM, NN)) || (st = tmp.alloc(&dS, 1)))
         return st;
     int status = 0;
     std::vector<int> hs(1);
     auto inv = [&](const cd *A, const cd *Sub, cd *o, int slice) -> int {
-        if (th <= 512) hipLaunchKernelGGL(k_inverse_batch<512>, dim3(1), dim3(th), 0, nullptr, N, Np, A, Sub, dW, o, dS);
-        else hipLaunchKernelGGL(k_inverse_batch<1024>, dim3(1), dim3(th), 0, nullptr, N, Np, A, Sub, dW, o, dS);
+        if (th <= 512) hipLaunchKernelGGL((k_inverse_batch<512, 32>), dim3(1), dim3(th), 0, nullptr, N, Np, A, Sub, dW, o, dS);
+        else hipLaunchKernelGGL((k_inverse_batch<1024, 16>), dim3(1), dim3(th), 0, nullptr, N, Np, A, Sub, dW, o, dS);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(hs.data(), dS, sizeof(int), hipMemcpyDeviceToHost));
         if (hs[0] && !status) status = slice;

@@ -336,3 +336,224 @@ __device__ __forceinline__ void copy_cols(cd *__restrict__ dst, const cd *__rest
         for (int r = threadIdx.x; r < Np; r += blockDim.x) dst[r + (size_t)col * Np] = sc[r];
     }
 }
+
+// ====================================================================================
+// Gauss-Jordan inversion, streaming version.
+//
+// Same pivot rule and arithmetic as gje_invert; reorganised so that the trailing
+// update -- the O(N^3) part -- needs no workgroup barrier:
+//  * each of the PB pivot steps of a panel takes ONE barrier: every wave publishes its
+//    best candidate (value, row index, and the row itself) plus the old content of row c,
+//    then every thread reads the winner from LDS; the LDS buffers alternate by step parity.
+//  * after the panel, the old pivot rows (all columns) are copied to a per-job scratch
+//    area T (PB x Np, column-major so a column block is contiguous) and the rows displaced
+//    by the interchanges are moved in place, so the update reads rows in natural order.
+//  * every wave then streams the column blocks on its own: it stages the T block in a
+//    wave-private LDS area, prefetches the next block into registers while it computes
+//    the current one, and writes its rows back.
+// ====================================================================================
+#ifndef NEGF_PB2
+#define NEGF_PB2 32   // panel width of the streaming kernel
+#endif
+#define NEGF_CB2 4    // columns per streamed block
+#define NEGF_MAXW 16  // waves per workgroup (1024 threads)
+
+template <int PB>
+struct GjeShared {
+    int perm[NEGF_MAXNP];
+    int ipiv[NEGF_MAXNP];
+    int sigma[NEGF_MAXNP];
+    int disp[2 * PB];
+    int ndisp;
+    int flag;
+    double rv[2][NEGF_MAXW];
+    int ri[2][NEGF_MAXW];
+    cd cand[2][NEGF_MAXW][PB];
+    cd rowC[2][PB];
+    cd Tw[NEGF_MAXW][PB * NEGF_CB2];
+    double red[NEGF_MAXW];
+};
+
+template <int PB, int J>
+__device__ __forceinline__ void gje_step2(cd (&P)[PB], const int k0, const int Np, const bool own,
+                                          GjeShared<PB> &s) {
+    const int r = threadIdx.x, lane = r & 63, w = r >> 6, nw = blockDim.x >> 6;
+    const int c = k0 + J, par = J & 1;
+    double v = -1.0;
+    int idx = 0x7fffffff;
+    if (own && r >= c) {
+        v = cabs1(P[J]);
+        if (!(v >= 0.0)) v = -0.5;   // NaN ranks below numbers, above ineligible rows
+        idx = r;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off, 64);
+        const int oi = __shfl_xor(idx, off, 64);
+        if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    }
+    if (lane == 0) { s.rv[par][w] = v; s.ri[par][w] = idx; }
+    if (r == idx) {
+#pragma unroll
+        for (int q = 0; q < PB; ++q) s.cand[par][w][q] = P[q];
+    }
+    if (r == c) {
+#pragma unroll
+        for (int q = 0; q < PB; ++q) s.rowC[par][q] = P[q];
+    }
+    __syncthreads();
+    double bv = s.rv[par][0];
+    int p = s.ri[par][0], wb = 0;
+    for (int i = 1; i < nw; ++i) {
+        const double ov = s.rv[par][i];
+        const int oi = s.ri[par][i];
+        if (ov > bv || (ov == bv && oi < p)) { bv = ov; p = oi; wb = i; }
+    }
+    if (p < c || p >= Np) p = c;
+    const cd *prow = (p == c) ? s.rowC[par] : s.cand[par][wb];
+    const cd inv = crecip(prow[J]);
+    if (r == 0) {
+        s.ipiv[c] = p;
+        if (p != c) {
+            const int t = s.perm[c];
+            s.perm[c] = s.perm[p];
+            s.perm[p] = t;
+        }
+        if (prow[J].x == 0.0 && prow[J].y == 0.0 && s.flag == 0) s.flag = c + 1;
+    }
+    if (r == c) {
+#pragma unroll
+        for (int q = 0; q < PB; ++q) P[q] = (q == J) ? inv : cmul(prow[q], inv);
+    } else if (own) {
+        if (r == p) {
+#pragma unroll
+            for (int q = 0; q < PB; ++q) P[q] = s.rowC[par][q];
+        }
+        const cd f = P[J];
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            if (q == J) P[q] = cmul(cmk(-f.x, -f.y), inv);
+            else P[q] = cfms(f, cmul(prow[q], inv), P[q]);
+        }
+    }
+}
+
+template <int PB, int... Js>
+__device__ __forceinline__ void gje_steps2(cd (&P)[PB], const int k0, const int Np, const bool own,
+                                           GjeShared<PB> &s, std::integer_sequence<int, Js...>) {
+    (gje_step2<PB, Js>(P, k0, Np, own, s), ...);
+}
+
+__device__ __forceinline__ double block_sum2(double v, double *red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t += red[i];
+    __syncthreads();
+    return t;
+}
+
+// In-place inversion of Z (Np x Np, Np a multiple of PB); on return
+// inverse[:, c] = Z[:, s.sigma[c]].  scratch: (2 PB) x Np complex per workgroup.
+// Returns the 1-based column of the first exactly-zero pivot, or 0.
+template <int PB>
+__device__ __forceinline__ int gje_invert2(cd *__restrict__ Z, const int Np, GjeShared<PB> &s,
+                                           cd *__restrict__ scratch) {
+    const int r = threadIdx.x, lane = r & 63, w = r >> 6;
+    const bool own = r < Np;
+    const size_t ld = (size_t)Np;
+    constexpr int CB = NEGF_CB2;
+    constexpr int TPL = PB * CB / 64;      // T entries staged per lane per block
+    cd *Tg = scratch;                      // old pivot rows: Tg[j + c PB]
+    cd *Dg = scratch + (size_t)PB * Np;    // displaced rows: Dg[d + c PB]
+    if (r == 0) s.flag = 0;
+    for (int k0 = 0; k0 < Np; k0 += PB) {
+        cd P[PB];
+#pragma unroll
+        for (int j = 0; j < PB; ++j) P[j] = own ? Z[r + (k0 + j) * ld] : cmk(0.0, 0.0);
+        for (int i = r; i < Np; i += blockDim.x) s.perm[i] = i;
+        if (r == 0) s.ndisp = 0;
+        __syncthreads();
+        gje_steps2<PB>(P, k0, Np, own, s, std::make_integer_sequence<int, PB>{});
+        __syncthreads();
+        if (own) {
+#pragma unroll
+            for (int j = 0; j < PB; ++j) Z[r + (k0 + j) * ld] = P[j];
+            if (r >= k0 + PB && s.perm[r] != r) s.disp[atomicAdd(&s.ndisp, 1)] = r;
+        }
+        __syncthreads();
+        // gather old pivot rows and displaced rows (every column but the panel's)
+        const int nd = s.ndisp;
+        const int rows = PB + nd;
+        for (int e = r; e < rows * Np; e += blockDim.x) {
+            const int q = e % rows, col = e / rows;
+            if (col >= k0 && col < k0 + PB) continue;
+            if (q < PB) Tg[q + (size_t)col * PB] = Z[s.perm[k0 + q] + col * ld];
+            else Dg[(q - PB) + (size_t)col * PB] = Z[s.perm[s.disp[q - PB]] + col * ld];
+        }
+        __syncthreads();
+        for (int e = r; e < nd * Np; e += blockDim.x) {
+            const int q = e % nd, col = e / nd;
+            if (col >= k0 && col < k0 + PB) continue;
+            Z[s.disp[q] + col * ld] = Dg[q + (size_t)col * PB];
+        }
+        __syncthreads();
+        // streamed rank-PB update of every other column, wave by wave
+        const bool pivrow = (r >= k0 && r < k0 + PB);
+        cd *tw = s.Tw[w];
+        int cb = (k0 == 0) ? PB : 0;
+        cd a[CB], tn[TPL];
+        if (cb < Np) {
+#pragma unroll
+            for (int cc = 0; cc < CB; ++cc) a[cc] = (own && !pivrow) ? Z[r + (cb + cc) * ld] : cmk(0.0, 0.0);
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) tn[t] = Tg[lane + 64 * t + (size_t)cb * PB];
+        }
+        while (cb < Np) {
+            int cbn = cb + CB;
+            if (cbn >= k0 && cbn < k0 + PB) cbn = k0 + PB;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) tw[lane + 64 * t] = tn[t];
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            cd an[CB];
+            if (cbn < Np) {
+#pragma unroll
+                for (int cc = 0; cc < CB; ++cc)
+                    an[cc] = (own && !pivrow) ? Z[r + (cbn + cc) * ld] : cmk(0.0, 0.0);
+#pragma unroll
+                for (int t = 0; t < TPL; ++t) tn[t] = Tg[lane + 64 * t + (size_t)cbn * PB];
+            }
+            if (own) {
+#pragma unroll
+                for (int cc = 0; cc < CB; ++cc) {
+                    cd acc = a[cc];
+#pragma unroll
+                    for (int j = 0; j < PB; ++j) acc = cfma(P[j], tw[j + cc * PB], acc);
+                    Z[r + (cb + cc) * ld] = acc;
+                }
+            }
+#pragma unroll
+            for (int cc = 0; cc < CB; ++cc) a[cc] = an[cc];
+            cb = cbn;
+        }
+        __syncthreads();
+    }
+    if (r == 0) {
+        for (int c = 0; c < Np; ++c) s.sigma[c] = c;
+        for (int c = Np - 1; c >= 0; --c) {
+            const int p = s.ipiv[c];
+            const int t = s.sigma[c];
+            s.sigma[c] = s.sigma[p];
+            s.sigma[p] = t;
+        }
+    }
+    __syncthreads();
+    return s.flag;
+}

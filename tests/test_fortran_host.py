@@ -84,7 +84,7 @@ def oracle_case(V_eV):
 
 @pytest.mark.gpu
 def test_fortran_host_end_to_end(tmp_path):
-    N_x, N_y, N_z = 16, 18, 5
+    N_x, N_y, N_z = 24, 28, 5      # 2 NGX <= N_x and NGY + 1 + N_y/2 <= N_y (NGX=12, NGY=13)
     V_eV = 0.3 * np.random.default_rng(4).standard_normal((N_x, N_y, N_z)) + 1.0
     inp = dict(CASE, if_transmission=True, if_density=True, if_LDOS=True, DUMP_FULL=True)
     write_case(tmp_path, inp, V_eV)
