@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -103,9 +104,19 @@ __device__ __forceinline__ void row_product(cd *__restrict__ Rn, const cd *__res
     __syncthreads();
 }
 
-template <int MAXT, int PB>
+// TWO: two threads per matrix row (gje_invert3); otherwise one (gje_invert2).
+template <int PB, bool TWO>
+using GjeSh = typename std::conditional<TWO, Gje3Shared<PB>, GjeShared<PB>>::type;
+
+template <int PB, bool TWO>
+__device__ __forceinline__ int gje_any(cd *Z, int Np, GjeSh<PB, TWO> &s, cd *scratch) {
+    if constexpr (TWO) return gje_invert3<PB>(Z, Np, s, scratch);
+    else return gje_invert2<PB>(Z, Np, s, scratch);
+}
+
+template <int MAXT, int PB, bool TWO>
 __global__ __launch_bounds__(MAXT) void k_transmission(TransArgs a) {
-    __shared__ GjeShared<PB> s;
+    __shared__ GjeSh<PB, TWO> s;
     __shared__ RowProdShared rs;
     const SysView &sv = a.sv;
     const int Np = sv.Np, ldR = a.nLmax;
@@ -134,7 +145,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission(TransArgs a) {
             // M = D_m - g_{m+1} (m = N_z: M = D_{N_z}), inverted in place
             build_block(M, sv, jv, m - 1, m == sv.Nz ? nullptr : S, s.sigma);
             __syncthreads();
-            if (gje_invert2<PB>(M, Np, s, scratch) && !bad) bad = m;
+            if (gje_any<PB, TWO>(M, Np, s, scratch) && !bad) bad = m;
             if (m == sv.Nz) {
                 // R = open rows of g_{N_z}
                 for (int e = threadIdx.x; e < nL * Np; e += blockDim.x) {
@@ -187,9 +198,9 @@ struct DiagArgs {
     int *status;
 };
 
-template <int MAXT, int PB>
+template <int MAXT, int PB, bool TWO>
 __global__ __launch_bounds__(MAXT) void k_diagonal(DiagArgs a) {
-    __shared__ GjeShared<PB> s;
+    __shared__ GjeSh<PB, TWO> s;
     const SysView &sv = a.sv;
     const int Np = sv.Np, N = sv.N, Nz = sv.Nz;
     const size_t NN = (size_t)Np * Np;
@@ -210,7 +221,7 @@ __global__ __launch_bounds__(MAXT) void k_diagonal(DiagArgs a) {
         for (int m = Nz; m >= 1; --m) {
             build_block(W, sv, jv, m - 1, m == Nz ? nullptr : G + (size_t)m * NN, nullptr);
             __syncthreads();
-            if (gje_invert2<PB>(W, Np, s, scratch) && !bad) bad = m;
+            if (gje_any<PB, TWO>(W, Np, s, scratch) && !bad) bad = m;
             copy_cols(m >= 2 ? G + (size_t)(m - 1) * NN : X, W, s.sigma, Np);
             __syncthreads();
         }
@@ -253,10 +264,10 @@ __global__ __launch_bounds__(MAXT) void k_diagonal(DiagArgs a) {
 // and the building blocks of the generic tri-diagonal solver).
 // inv[b] = (A[b] - (Sub ? Sub[b] : 0))^{-1} for count matrices of N x N (ld N on both
 // sides); the work copy is padded to Np.
-template <int MAXT, int PB>
+template <int MAXT, int PB, bool TWO>
 __global__ __launch_bounds__(MAXT) void k_inverse_batch(int N, int Np, const cd *A, const cd *Sub,
                                                         cd *work, cd *inv, int *status) {
-    __shared__ GjeShared<PB> s;
+    __shared__ GjeSh<PB, TWO> s;
     const int b = blockIdx.x;
     const size_t NN = (size_t)Np * Np;
     cd *Wk = work + b * (NN + 2 * (size_t)PB * Np);
@@ -272,7 +283,7 @@ __global__ __launch_bounds__(MAXT) void k_inverse_batch(int N, int Np, const cd 
             Wk[r + (size_t)col * Np] = v;
         }
     __syncthreads();
-    const int zp = gje_invert2<PB>(Wk, Np, s, Wk + NN);
+    const int zp = gje_any<PB, TWO>(Wk, Np, s, Wk + NN);
     cd *ob = inv + (size_t)b * N * N;
     for (int col = 0; col < N; ++col) {
         const cd *sc = Wk + (size_t)s.sigma[col] * Np;
@@ -519,6 +530,14 @@ int padded_size(int N) {
 }
 int panel_width(int Np) { return Np <= 512 ? 32 : 16; }
 
+// Kernel shape per padded size: two threads per row (640- or 1024-thread groups) up to
+// Np = 512, one thread per row with panel 16 above.
+enum Shape { SHAPE_640, SHAPE_1024_TWO, SHAPE_1024_ONE };
+Shape shape_for(int Np) { return Np <= 320 ? SHAPE_640 : (Np <= 512 ? SHAPE_1024_TWO : SHAPE_1024_ONE); }
+int threads_for_shape(int Np) {
+    return shape_for(Np) == SHAPE_1024_ONE ? std::min(1024, round_up(Np, 64)) : round_up(2 * Np, 64);
+}
+
 // Per-job host tables: kinetic/2 per k-point, lead terms and trace weights per job.
 struct JobTables {
     std::vector<cd> z, sigL, sigR;
@@ -610,6 +629,21 @@ int slots_for(negf_ctx *c, const void *kernel, int threads, int njobs, size_t pe
 }
 
 }  // namespace
+
+static void launch_inverse(int Np, int count, hipStream_t st, int N, const cd *A, const cd *Sub, cd *work,
+                           cd *inv, int *status) {
+    const int th = threads_for_shape(Np);
+    switch (shape_for(Np)) {
+    case SHAPE_640:
+        hipLaunchKernelGGL((k_inverse_batch<640, 32, true>), dim3(count), dim3(th), 0, st, N, Np, A, Sub, work, inv, status);
+        break;
+    case SHAPE_1024_TWO:
+        hipLaunchKernelGGL((k_inverse_batch<1024, 32, true>), dim3(count), dim3(th), 0, st, N, Np, A, Sub, work, inv, status);
+        break;
+    default:
+        hipLaunchKernelGGL((k_inverse_batch<1024, 16, false>), dim3(count), dim3(th), 0, st, N, Np, A, Sub, work, inv, status);
+    }
+}
 
 extern "C" {
 
@@ -761,10 +795,13 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
         (st = tmp.upload(&d_gr, gRv.data(), gRv.size(), c->stream)) ||
         (st = tmp.alloc(&d_tau, nj)) || (st = tmp.alloc(&d_st, nj)))
         return st;
-    const int threads = threads_for(c->Np);
+    const int threads = threads_for_shape(c->Np);
+    const Shape shp = shape_for(c->Np);
     const size_t per_slot =
         ((size_t)2 * c->Np * c->Np + (size_t)2 * nLmax * c->Np + (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
-    const void *kfn = threads <= 512 ? (const void *)k_transmission<512, 32> : (const void *)k_transmission<1024, 16>;
+    const void *kfn = shp == SHAPE_640 ? (const void *)k_transmission<640, 32, true>
+                    : shp == SHAPE_1024_TWO ? (const void *)k_transmission<1024, 32, true>
+                                            : (const void *)k_transmission<1024, 16, false>;
     const int slots = slots_for(c, kfn, threads, nj, per_slot);
     if ((st = ensure_ws(c, per_slot * slots))) return st;
     a.z = d_z;
@@ -784,8 +821,9 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
     a.tau = d_tau;
     a.status = d_st;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    if (threads <= 512) hipLaunchKernelGGL((k_transmission<512, 32>), dim3(slots), dim3(threads), 0, c->stream, a);
-    else hipLaunchKernelGGL((k_transmission<1024, 16>), dim3(slots), dim3(threads), 0, c->stream, a);
+    if (shp == SHAPE_640) hipLaunchKernelGGL((k_transmission<640, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
+    else if (shp == SHAPE_1024_TWO) hipLaunchKernelGGL((k_transmission<1024, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_transmission<1024, 16, false>), dim3(slots), dim3(threads), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     std::vector<double> h_tau(nj);
@@ -842,10 +880,13 @@ static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &e
         if ((st = tmp.alloc(&d_out, nout))) return st;
         HIPCHK(hipMemsetAsync(d_out, 0, nout * sizeof(cd), c->stream));
     }
-    const int threads = threads_for(c->Np);
+    const int threads = threads_for_shape(c->Np);
+    const Shape shp = shape_for(c->Np);
     const size_t per_slot =
         ((size_t)(c->Nz + 2) * c->Np * c->Np + (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
-    const void *kfn = threads <= 512 ? (const void *)k_diagonal<512, 32> : (const void *)k_diagonal<1024, 16>;
+    const void *kfn = shp == SHAPE_640 ? (const void *)k_diagonal<640, 32, true>
+                    : shp == SHAPE_1024_TWO ? (const void *)k_diagonal<1024, 32, true>
+                                            : (const void *)k_diagonal<1024, 16, false>;
     const int slots = slots_for(c, kfn, threads, nj, per_slot);
     if ((st = ensure_ws(c, per_slot * slots))) return st;
     a.z = d_z;
@@ -859,8 +900,9 @@ static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &e
     a.out = d_out;
     a.status = d_st;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    if (threads <= 512) hipLaunchKernelGGL((k_diagonal<512, 32>), dim3(slots), dim3(threads), 0, c->stream, a);
-    else hipLaunchKernelGGL((k_diagonal<1024, 16>), dim3(slots), dim3(threads), 0, c->stream, a);
+    if (shp == SHAPE_640) hipLaunchKernelGGL((k_diagonal<640, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
+    else if (shp == SHAPE_1024_TWO) hipLaunchKernelGGL((k_diagonal<1024, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_diagonal<1024, 16, false>), dim3(slots), dim3(threads), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     std::vector<int> h_st(nj);
@@ -1070,12 +1112,7 @@ int negf_inverse_batch(int N, int count, const double *A, double *inv) {
         (st = tmp.alloc(&dW, ((size_t)Np * Np + 2 * (size_t)panel_width(Np) * Np) * count)) || (st = tmp.alloc(&dI, nn)) ||
         (st = tmp.alloc(&dS, count)))
         return st;
-    if (threads_for(Np) <= 512)
-        hipLaunchKernelGGL((k_inverse_batch<512, 32>), dim3(count), dim3(threads_for(Np)), 0, nullptr, N, Np, dA,
-                           (const cd *)nullptr, dW, dI, dS);
-    else
-        hipLaunchKernelGGL((k_inverse_batch<1024, 16>), dim3(count), dim3(threads_for(Np)), 0, nullptr, N, Np, dA,
-                           (const cd *)nullptr, dW, dI, dS);
+    launch_inverse(Np, count, nullptr, N, dA, nullptr, dW, dI, dS);
     HIPCHK(hipGetLastError());
     std::vector<int> hs(count);
     HIPCHK(hipMemcpy(inv, dI, nn * sizeof(cd), hipMemcpyDeviceToHost));
@@ -1139,8 +1176,7 @@ int negf_rgf_tri_solve(int N, int N_z, const double *G_inv_D, int mode, double *
     int status = 0;
     std::vector<int> hs(1);
     auto inv = [&](const cd *A, const cd *Sub, cd *o, int slice) -> int {
-        if (th <= 512) hipLaunchKernelGGL((k_inverse_batch<512, 32>), dim3(1), dim3(th), 0, nullptr, N, Np, A, Sub, dW, o, dS);
-        else hipLaunchKernelGGL((k_inverse_batch<1024, 16>), dim3(1), dim3(th), 0, nullptr, N, Np, A, Sub, dW, o, dS);
+        launch_inverse(Np, 1, nullptr, N, A, Sub, dW, o, dS);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(hs.data(), dS, sizeof(int), hipMemcpyDeviceToHost));
         if (hs[0] && !status) status = slice;

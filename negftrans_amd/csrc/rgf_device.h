@@ -299,32 +299,53 @@ struct SysView {
 // operation order of E_minus_H_construction (src/grid.f90:291-331) with
 // H_k(i,j) = V(nx_i - nx_j, ny_i - ny_j, k) + delta_ij kin_i (src/grid.f90:248-259).
 __device__ __forceinline__ void build_block(cd *__restrict__ M, const SysView &sv, const JobView &jv,
-                            const int slice, const cd *__restrict__ S, const int *sig) {
+                                            const int slice, const cd *__restrict__ S, const int *sig) {
     const int Np = sv.Np, N = sv.N;
     const int W = 2 * sv.NGX + 1;
-    const cd *vs = sv.vrec + (size_t)slice * W * (2 * sv.NGY + 1);
-    for (int col = 0; col < Np; ++col) {
-        const cd *scol = S ? S + (size_t)(sig ? sig[col] : col) * Np : nullptr;
-        for (int r = threadIdx.x; r < Np; r += blockDim.x) {
-            cd v;
-            if (r < N && col < N) {
-                cd h = vs[(sv.nx[r] - sv.nx[col] + sv.NGX) + W * (sv.ny[r] - sv.ny[col] + sv.NGY)];
-                if (r == col) h.x += jv.kinh[r];
-                cd d = cmk(-h.x, -h.y);
-                if (r == col) d = cadd(d, jv.z);
-                d.x *= sv.scale;
-                d.y *= sv.scale;
-                if (r == col) {
-                    d.x -= 2.0;
-                    if (slice == 0) d = cadd(d, jv.sigL[r]);
-                    if (slice == sv.Nz - 1) d = cadd(d, jv.sigR[r]);
-                }
-                if (scol) d = csub(d, scol[r]);
-                v = d;
-            } else {
-                v = cmk(r == col ? 1.0 : 0.0, 0.0);
+    const cd *__restrict__ vs = sv.vrec + (size_t)slice * W * (2 * sv.NGY + 1);
+    constexpr int CBB = 8;
+    for (int r = threadIdx.x; r < Np; r += blockDim.x) {
+        const bool rin = r < N;
+        const int nxr = rin ? sv.nx[r] : 0, nyr = rin ? sv.ny[r] : 0;
+        const double kin = rin ? jv.kinh[r] : 0.0;
+        cd dg = cmk(0.0, 0.0);           // diagonal additions after scaling
+        if (rin) {
+            if (slice == 0) dg = cadd(dg, jv.sigL[r]);
+            if (slice == sv.Nz - 1) dg = cadd(dg, jv.sigR[r]);
+        }
+        for (int c0 = 0; c0 < Np; c0 += CBB) {
+            cd hv[CBB], sv_[CBB];
+#pragma unroll
+            for (int q = 0; q < CBB; ++q) {
+                const int col = c0 + q;
+                const bool in = rin && col < N;
+                int ix = 0;
+                if (in) ix = (nxr - sv.nx[col] + sv.NGX) + W * (nyr - sv.ny[col] + sv.NGY);
+                hv[q] = in ? vs[ix] : cmk(0.0, 0.0);
+                sv_[q] = S ? S[r + (size_t)(sig ? sig[col] : col) * Np] : cmk(0.0, 0.0);
             }
-            M[r + (size_t)col * Np] = v;
+#pragma unroll
+            for (int q = 0; q < CBB; ++q) {
+                const int col = c0 + q;
+                cd v;
+                if (rin && col < N) {
+                    cd h = hv[q];
+                    if (r == col) h.x += kin;
+                    cd d = cmk(-h.x, -h.y);
+                    if (r == col) d = cadd(d, jv.z);
+                    d.x *= sv.scale;
+                    d.y *= sv.scale;
+                    if (r == col) {
+                        d.x -= 2.0;
+                        d = cadd(d, dg);
+                    }
+                    if (S) d = csub(d, sv_[q]);
+                    v = d;
+                } else {
+                    v = cmk(r == col ? 1.0 : 0.0, 0.0);
+                }
+                M[r + (size_t)col * Np] = v;
+            }
         }
     }
 }
@@ -374,25 +395,61 @@ struct GjeShared {
     double red[NEGF_MAXW];
 };
 
-template <int PB, int J>
-__device__ __forceinline__ void gje_step2(cd (&P)[PB], const int k0, const int Np, const bool own,
-                                          GjeShared<PB> &s) {
+// Wave-wide max of a double with DPP (row-local shuffles and row broadcasts), result
+// valid in every lane.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_max_step(double v) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWMASK, 0xf, false);
+    const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWMASK, 0xf, false);
+    const double o = __hiloint2double(ohi, olo);
+    return o > v ? o : v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+    v = dpp_max_step<0xB1, 0xf>(v);    // quad_perm [1,0,3,2]
+    v = dpp_max_step<0x4E, 0xf>(v);    // quad_perm [2,3,0,1]
+    v = dpp_max_step<0x141, 0xf>(v);   // row_half_mirror
+    v = dpp_max_step<0x140, 0xf>(v);   // row_mirror
+    v = dpp_max_step<0x142, 0xa>(v);   // row_bcast:15
+    v = dpp_max_step<0x143, 0xc>(v);   // row_bcast:31
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+
+// Value of lane L (compile-time) of a complex register, as a wave-uniform scalar.
+__device__ __forceinline__ cd wave_bcast(cd v, int L) {
+    const int xl = __builtin_amdgcn_readlane(__double2loint(v.x), L);
+    const int xh = __builtin_amdgcn_readlane(__double2hiint(v.x), L);
+    const int yl = __builtin_amdgcn_readlane(__double2loint(v.y), L);
+    const int yh = __builtin_amdgcn_readlane(__double2hiint(v.y), L);
+    return cmk(__hiloint2double(xh, xl), __hiloint2double(yh, yl));
+}
+
+// One Gauss-Jordan step (column c = k0 + J) of the register-resident panel.  J is a
+// runtime (wave-uniform) index: the panel registers are addressed through unrolled
+// selects, so the PB steps share one copy of the code instead of PB unrolled copies
+// (the unrolled form overflows the instruction cache).
+template <int PB>
+__device__ __forceinline__ void gje_step2(cd (&P)[PB], const int J, const int k0, const int Np,
+                                          const bool own, GjeShared<PB> &s) {
     const int r = threadIdx.x, lane = r & 63, w = r >> 6, nw = blockDim.x >> 6;
     const int c = k0 + J, par = J & 1;
-    double v = -1.0;
-    int idx = 0x7fffffff;
-    if (own && r >= c) {
-        v = cabs1(P[J]);
-        if (!(v >= 0.0)) v = -0.5;   // NaN ranks below numbers, above ineligible rows
-        idx = r;
-    }
+    cd pj = cmk(0.0, 0.0);
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(v, off, 64);
-        const int oi = __shfl_xor(idx, off, 64);
-        if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    for (int q = 0; q < PB; ++q)
+        if (q == J) pj = P[q];
+    double v = -1.0;
+    if (own && r >= c) {
+        v = cabs1(pj);
+        if (!(v >= 0.0)) v = -0.5;   // NaN ranks below numbers, above ineligible rows
     }
-    if (lane == 0) { s.rv[par][w] = v; s.ri[par][w] = idx; }
+    // wave argmax: max value, then the lowest lane holding it (rows are lanes in order,
+    // so the lowest lane is the lowest row index, as izamax picks the first maximum)
+    const double vmax = wave_max(v);
+    const unsigned long long hit = __ballot(v == vmax);
+    const int idx = (vmax >= 0.0 || vmax == -0.5) ? (w << 6) + (__ffsll((long long)hit) - 1) : 0x7fffffff;
+    if (lane == 0) { s.rv[par][w] = vmax; s.ri[par][w] = idx; }
     if (r == idx) {
 #pragma unroll
         for (int q = 0; q < PB; ++q) s.cand[par][w][q] = P[q];
@@ -411,7 +468,8 @@ __device__ __forceinline__ void gje_step2(cd (&P)[PB], const int k0, const int N
     }
     if (p < c || p >= Np) p = c;
     const cd *prow = (p == c) ? s.rowC[par] : s.cand[par][wb];
-    const cd inv = crecip(prow[J]);
+    const cd pv = prow[J];
+    const cd inv = crecip(pv);
     if (r == 0) {
         s.ipiv[c] = p;
         if (p != c) {
@@ -419,29 +477,31 @@ __device__ __forceinline__ void gje_step2(cd (&P)[PB], const int k0, const int N
             s.perm[c] = s.perm[p];
             s.perm[p] = t;
         }
-        if (prow[J].x == 0.0 && prow[J].y == 0.0 && s.flag == 0) s.flag = c + 1;
+        if (pv.x == 0.0 && pv.y == 0.0 && s.flag == 0) s.flag = c + 1;
     }
-    if (r == c) {
-#pragma unroll
-        for (int q = 0; q < PB; ++q) P[q] = (q == J) ? inv : cmul(prow[q], inv);
-    } else if (own) {
-        if (r == p) {
+    // Row r <- row r - (P[r][J] / pivot) * pivot row; the pivot row itself becomes
+    // pivot row / pivot (same form with base 0 and multiplier -1/pivot); column J holds
+    // -multiplier.  One complex FMA per entry.
+    if (own) {
+        const bool piv = (r == c);
+        const bool takeC = (r == p && r != c);
+        cd mr = pj;
+        if (takeC) {              // this row receives the old content of row c
+            mr = s.rowC[par][J];
 #pragma unroll
             for (int q = 0; q < PB; ++q) P[q] = s.rowC[par][q];
         }
-        const cd f = P[J];
+        if (piv) {
+#pragma unroll
+            for (int q = 0; q < PB; ++q) P[q] = cmk(0.0, 0.0);
+        }
+        const cd f = piv ? cmk(-inv.x, -inv.y) : cmul(mr, inv);
 #pragma unroll
         for (int q = 0; q < PB; ++q) {
-            if (q == J) P[q] = cmul(cmk(-f.x, -f.y), inv);
-            else P[q] = cfms(f, cmul(prow[q], inv), P[q]);
+            const cd nv = cfms(f, prow[q], P[q]);
+            P[q] = (q == J) ? cmk(-f.x, -f.y) : nv;
         }
     }
-}
-
-template <int PB, int... Js>
-__device__ __forceinline__ void gje_steps2(cd (&P)[PB], const int k0, const int Np, const bool own,
-                                           GjeShared<PB> &s, std::integer_sequence<int, Js...>) {
-    (gje_step2<PB, Js>(P, k0, Np, own, s), ...);
 }
 
 __device__ __forceinline__ double block_sum2(double v, double *red) {
@@ -463,6 +523,7 @@ __device__ __forceinline__ double block_sum2(double v, double *red) {
 template <int PB>
 __device__ __forceinline__ int gje_invert2(cd *__restrict__ Z, const int Np, GjeShared<PB> &s,
                                            cd *__restrict__ scratch) {
+    static_assert(NEGF_CB2 * PB % 64 == 0, "T block must split evenly over a wave");
     const int r = threadIdx.x, lane = r & 63, w = r >> 6;
     const bool own = r < Np;
     const size_t ld = (size_t)Np;
@@ -478,7 +539,8 @@ __device__ __forceinline__ int gje_invert2(cd *__restrict__ Z, const int Np, Gje
         for (int i = r; i < Np; i += blockDim.x) s.perm[i] = i;
         if (r == 0) s.ndisp = 0;
         __syncthreads();
-        gje_steps2<PB>(P, k0, Np, own, s, std::make_integer_sequence<int, PB>{});
+#pragma unroll 1
+        for (int J = 0; J < PB; ++J) gje_step2<PB>(P, J, k0, Np, own, s);
         __syncthreads();
         if (own) {
 #pragma unroll
@@ -489,11 +551,29 @@ __device__ __forceinline__ int gje_invert2(cd *__restrict__ Z, const int Np, Gje
         // gather old pivot rows and displaced rows (every column but the panel's)
         const int nd = s.ndisp;
         const int rows = PB + nd;
-        for (int e = r; e < rows * Np; e += blockDim.x) {
-            const int q = e % rows, col = e / rows;
-            if (col >= k0 && col < k0 + PB) continue;
-            if (q < PB) Tg[q + (size_t)col * PB] = Z[s.perm[k0 + q] + col * ld];
-            else Dg[(q - PB) + (size_t)col * PB] = Z[s.perm[s.disp[q - PB]] + col * ld];
+        {
+            constexpr int GB = 8;
+            const int total = rows * Np;
+            for (int e0 = r; e0 < total; e0 += GB * blockDim.x) {
+                cd gv[GB];
+#pragma unroll
+                for (int g = 0; g < GB; ++g) {
+                    const int e = e0 + g * blockDim.x;
+                    const int q = e % rows, col = e / rows;
+                    const bool ok = e < total && !(col >= k0 && col < k0 + PB);
+                    const int src = ok ? (q < PB ? s.perm[k0 + q] : s.perm[s.disp[q - PB]]) : 0;
+                    gv[g] = ok ? Z[src + col * ld] : cmk(0.0, 0.0);
+                }
+#pragma unroll
+                for (int g = 0; g < GB; ++g) {
+                    const int e = e0 + g * blockDim.x;
+                    const int q = e % rows, col = e / rows;
+                    if (e < total && !(col >= k0 && col < k0 + PB)) {
+                        if (q < PB) Tg[q + (size_t)col * PB] = gv[g];
+                        else Dg[(q - PB) + (size_t)col * PB] = gv[g];
+                    }
+                }
+            }
         }
         __syncthreads();
         for (int e = r; e < nd * Np; e += blockDim.x) {
@@ -504,7 +584,6 @@ __device__ __forceinline__ int gje_invert2(cd *__restrict__ Z, const int Np, Gje
         __syncthreads();
         // streamed rank-PB update of every other column, wave by wave
         const bool pivrow = (r >= k0 && r < k0 + PB);
-        cd *tw = s.Tw[w];
         int cb = (k0 == 0) ? PB : 0;
         cd a[CB], tn[TPL];
         if (cb < Np) {
@@ -516,12 +595,9 @@ __device__ __forceinline__ int gje_invert2(cd *__restrict__ Z, const int Np, Gje
         while (cb < Np) {
             int cbn = cb + CB;
             if (cbn >= k0 && cbn < k0 + PB) cbn = k0 + PB;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
+            cd tc[TPL];
 #pragma unroll
-            for (int t = 0; t < TPL; ++t) tw[lane + 64 * t] = tn[t];
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
+            for (int t = 0; t < TPL; ++t) tc[t] = tn[t];
             cd an[CB];
             if (cbn < Np) {
 #pragma unroll
@@ -530,17 +606,21 @@ __device__ __forceinline__ int gje_invert2(cd *__restrict__ Z, const int Np, Gje
 #pragma unroll
                 for (int t = 0; t < TPL; ++t) tn[t] = Tg[lane + 64 * t + (size_t)cbn * PB];
             }
-            if (own) {
+            // T(j, cc) sits in lane (j + cc PB) % 64, register (j + cc PB) / 64 of tc;
+            // readlane turns it into a scalar operand of the FMAs.
 #pragma unroll
-                for (int cc = 0; cc < CB; ++cc) {
-                    cd acc = a[cc];
+            for (int cc = 0; cc < CB; ++cc) {
+                cd p4[4] = {a[cc], cmk(0.0, 0.0), cmk(0.0, 0.0), cmk(0.0, 0.0)};
 #pragma unroll
-                    for (int j = 0; j < PB; ++j) acc = cfma(P[j], tw[j + cc * PB], acc);
-                    Z[r + (cb + cc) * ld] = acc;
+                for (int j = 0; j < PB; ++j) {
+                    const int e = j + cc * PB;
+                    p4[j & 3] = cfma(P[j], wave_bcast(tc[e >> 6], e & 63), p4[j & 3]);
                 }
+                if (own) Z[r + (cb + cc) * ld] = cadd(cadd(p4[0], p4[1]), cadd(p4[2], p4[3]));
             }
 #pragma unroll
             for (int cc = 0; cc < CB; ++cc) a[cc] = an[cc];
+            cb = cbn;
             cb = cbn;
         }
         __syncthreads();
@@ -552,6 +632,244 @@ __device__ __forceinline__ int gje_invert2(cd *__restrict__ Z, const int Np, Gje
             const int t = s.sigma[c];
             s.sigma[c] = s.sigma[p];
             s.sigma[p] = t;
+        }
+    }
+    __syncthreads();
+    return s.flag;
+}
+
+// ====================================================================================
+// Gauss-Jordan inversion with two threads per row.
+//
+// Same algorithm and pivot rule as gje_invert2, laid out for latency hiding: thread
+// t owns half h = t & 1 of the panel row r = t >> 1 (PB/2 columns), so a workgroup has
+// 2 Np threads -- twice the waves per SIMD at the same panel width -- and each thread
+// keeps half the registers.  The two halves of a row sit in adjacent lanes; partial
+// sums and the pivot-column value are exchanged with a DPP lane swap.
+// ====================================================================================
+template <int PB>
+struct Gje3Shared {
+    int perm[NEGF_MAXNP];
+    int ipiv[NEGF_MAXNP];
+    int sigma[NEGF_MAXNP];
+    int disp[2 * PB];
+    int ndisp;
+    int flag;
+    double rv[2][NEGF_MAXW];
+    int ri[2][NEGF_MAXW];
+    cd cand[2][NEGF_MAXW][PB];
+    cd rowC[2][PB];
+    cd Tw[NEGF_MAXW][PB * NEGF_CB2];
+    double red[NEGF_MAXW];
+};
+
+// Value held by the partner lane (t ^ 1).
+__device__ __forceinline__ double lane_swap(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0xB1, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0xB1, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ cd lane_swap(cd v) { return cmk(lane_swap(v.x), lane_swap(v.y)); }
+
+template <int PB>
+__device__ __forceinline__ void gje3_step(cd (&P)[PB / 2], const int J, const int k0, const int Np,
+                                          const bool own, const int r, const int h, Gje3Shared<PB> &s) {
+    constexpr int HB = PB / 2;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = blockDim.x >> 6;
+    const int c = k0 + J, par = J & 1;
+    const int jh = J / HB, jq = J % HB;
+    cd pj = cmk(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < HB; ++q)
+        if (q == jq) pj = P[q];
+    const cd pjo = lane_swap(pj);
+    const cd pjr = (h == jh) ? pj : pjo;           // P[r][J], known to both lanes of row r
+    double v = -1.0;
+    if (own && r >= c && h == 0) {
+        v = cabs1(pjr);
+        if (!(v >= 0.0)) v = -0.5;
+    }
+    const double vmax = wave_max(v);
+    const unsigned long long hit = __ballot(v == vmax);
+    const int idx = (vmax >= 0.0 || vmax == -0.5) ? (((w << 6) + (__ffsll((long long)hit) - 1)) >> 1) : 0x7fffffff;
+    if (lane == 0) { s.rv[par][w] = vmax; s.ri[par][w] = idx; }
+    if (r == idx) {
+#pragma unroll
+        for (int q = 0; q < HB; ++q) s.cand[par][w][h * HB + q] = P[q];
+    }
+    if (r == c) {
+#pragma unroll
+        for (int q = 0; q < HB; ++q) s.rowC[par][h * HB + q] = P[q];
+    }
+    __syncthreads();
+    double bv = s.rv[par][0];
+    int p = s.ri[par][0], wb = 0;
+    for (int i = 1; i < nw; ++i) {
+        const double ov = s.rv[par][i];
+        const int oi = s.ri[par][i];
+        if (ov > bv || (ov == bv && oi < p)) { bv = ov; p = oi; wb = i; }
+    }
+    if (p < c || p >= Np) p = c;
+    const cd *prow = (p == c) ? s.rowC[par] : s.cand[par][wb];
+    const cd pv = prow[J];
+    const cd inv = crecip(pv);
+    if (t == 0) {
+        s.ipiv[c] = p;
+        if (p != c) {
+            const int tt = s.perm[c];
+            s.perm[c] = s.perm[p];
+            s.perm[p] = tt;
+        }
+        if (pv.x == 0.0 && pv.y == 0.0 && s.flag == 0) s.flag = c + 1;
+    }
+    if (own) {
+        const bool piv = (r == c);
+        const bool takeC = (r == p && r != c);
+        cd mr = pjr;
+        if (takeC) {
+            mr = s.rowC[par][J];
+#pragma unroll
+            for (int q = 0; q < HB; ++q) P[q] = s.rowC[par][h * HB + q];
+        }
+        if (piv) {
+#pragma unroll
+            for (int q = 0; q < HB; ++q) P[q] = cmk(0.0, 0.0);
+        }
+        const cd f = piv ? cmk(-inv.x, -inv.y) : cmul(mr, inv);
+        const cd *ph = prow + h * HB;
+#pragma unroll
+        for (int q = 0; q < HB; ++q) {
+            const cd nv = cfms(f, ph[q], P[q]);
+            P[q] = (h * HB + q == J) ? cmk(-f.x, -f.y) : nv;
+        }
+    }
+}
+
+// In-place inversion of Z (Np x Np, Np a multiple of PB) by a workgroup of 2 * Np
+// threads (rounded up to whole waves); on return inverse[:, c] = Z[:, s.sigma[c]].
+// scratch: (2 PB) x Np complex per workgroup.  Returns the 1-based column of the first
+// exactly-zero pivot, or 0.
+template <int PB>
+__device__ __forceinline__ int gje_invert3(cd *__restrict__ Z, const int Np, Gje3Shared<PB> &s,
+                                           cd *__restrict__ scratch) {
+    constexpr int HB = PB / 2;
+    constexpr int CB = NEGF_CB2;
+    static_assert(CB % 2 == 0 && PB * CB == 128, "block shape: two T entries per lane");
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int r = t >> 1, h = t & 1;
+    const bool own = r < Np;
+    const size_t ld = (size_t)Np;
+    cd *Tg = scratch;                      // old pivot rows: Tg[j + c PB]
+    cd *Dg = scratch + (size_t)PB * Np;    // displaced rows: Dg[d + c PB]
+    if (t == 0) s.flag = 0;
+    for (int k0 = 0; k0 < Np; k0 += PB) {
+        cd P[HB];
+#pragma unroll
+        for (int j = 0; j < HB; ++j) P[j] = own ? Z[r + (k0 + h * HB + j) * ld] : cmk(0.0, 0.0);
+        for (int i = t; i < Np; i += blockDim.x) s.perm[i] = i;
+        if (t == 0) s.ndisp = 0;
+        __syncthreads();
+#pragma unroll 1
+        for (int J = 0; J < PB; ++J) gje3_step<PB>(P, J, k0, Np, own, r, h, s);
+        __syncthreads();
+        if (own) {
+#pragma unroll
+            for (int j = 0; j < HB; ++j) Z[r + (k0 + h * HB + j) * ld] = P[j];
+            if (h == 0 && r >= k0 + PB && s.perm[r] != r) s.disp[atomicAdd(&s.ndisp, 1)] = r;
+        }
+        __syncthreads();
+        const int nd = s.ndisp;
+        const int rows = PB + nd;
+        {
+            constexpr int GB = 4;
+            const int total = rows * Np;
+            for (int e0 = t; e0 < total; e0 += GB * blockDim.x) {
+                cd gv[GB];
+#pragma unroll
+                for (int g = 0; g < GB; ++g) {
+                    const int e = e0 + g * blockDim.x;
+                    const int q = e % rows, col = e / rows;
+                    const bool ok = e < total && !(col >= k0 && col < k0 + PB);
+                    const int src = ok ? (q < PB ? s.perm[k0 + q] : s.perm[s.disp[q - PB]]) : 0;
+                    gv[g] = ok ? Z[src + col * ld] : cmk(0.0, 0.0);
+                }
+#pragma unroll
+                for (int g = 0; g < GB; ++g) {
+                    const int e = e0 + g * blockDim.x;
+                    const int q = e % rows, col = e / rows;
+                    if (e < total && !(col >= k0 && col < k0 + PB)) {
+                        if (q < PB) Tg[q + (size_t)col * PB] = gv[g];
+                        else Dg[(q - PB) + (size_t)col * PB] = gv[g];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int e = t; e < nd * Np; e += blockDim.x) {
+            const int q = e % nd, col = e / nd;
+            if (col >= k0 && col < k0 + PB) continue;
+            Z[s.disp[q] + col * ld] = Dg[q + (size_t)col * PB];
+        }
+        __syncthreads();
+        // streamed rank-PB update: lane h accumulates its half of the panel columns for
+        // the CB columns of a block, the partner's partial sums arrive through a lane
+        // swap, and lane h stores the block columns cc with cc % 2 == h.
+        const bool pivrow = (r >= k0 && r < k0 + PB);
+        const bool ld_ok = own && !pivrow;
+        cd *tw = s.Tw[w];
+        int cb = (k0 == 0) ? PB : 0;
+        // lane h owns block columns h, h+2, ...: a[i] is the old value of column 2i+h
+        // (the two staged T entries of a lane are named variables: as an array the
+        // compiler placed them in scratch, which serialised every prefetch)
+        cd a[CB / 2], tn0 = cmk(0.0, 0.0), tn1 = cmk(0.0, 0.0);
+        if (cb < Np) {
+#pragma unroll
+            for (int i = 0; i < CB / 2; ++i) a[i] = ld_ok ? Z[r + (cb + 2 * i + h) * ld] : cmk(0.0, 0.0);
+            tn0 = Tg[lane + (size_t)cb * PB];
+            tn1 = Tg[lane + 64 + (size_t)cb * PB];
+        }
+        while (cb < Np) {
+            int cbn = cb + CB;
+            if (cbn >= k0 && cbn < k0 + PB) cbn = k0 + PB;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            tw[lane] = tn0;
+            tw[lane + 64] = tn1;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            cd an[CB / 2];
+            if (cbn < Np) {
+#pragma unroll
+                for (int i = 0; i < CB / 2; ++i) an[i] = ld_ok ? Z[r + (cbn + 2 * i + h) * ld] : cmk(0.0, 0.0);
+                tn0 = Tg[lane + (size_t)cbn * PB];
+                tn1 = Tg[lane + 64 + (size_t)cbn * PB];
+            }
+            // four partial sums per column (j mod 4) are independent FMA chains, so
+            // consecutive FMAs do not wait on each other
+#pragma unroll
+            for (int cc = 0; cc < CB; ++cc) {
+                const cd *tcol = tw + cc * PB + h * HB;
+                cd p4[4] = {cmk(0.0, 0.0), cmk(0.0, 0.0), cmk(0.0, 0.0), cmk(0.0, 0.0)};
+#pragma unroll
+                for (int j = 0; j < HB; ++j) p4[j & 3] = cfma(P[j], tcol[j], p4[j & 3]);
+                const cd sacc = cadd(cadd(p4[0], p4[1]), cadd(p4[2], p4[3]));
+                // a[cc >> 1] is this lane's base only when cc % 2 == h, the one case stored
+                const cd tot = cadd(cadd(a[cc >> 1], sacc), lane_swap(sacc));
+                if (own && (cc & 1) == h) Z[r + (cb + cc) * ld] = tot;
+            }
+#pragma unroll
+            for (int i = 0; i < CB / 2; ++i) a[i] = an[i];
+            cb = cbn;
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        for (int c = 0; c < Np; ++c) s.sigma[c] = c;
+        for (int c = Np - 1; c >= 0; --c) {
+            const int p = s.ipiv[c];
+            const int tt = s.sigma[c];
+            s.sigma[c] = s.sigma[p];
+            s.sigma[p] = tt;
         }
     }
     __syncthreads();
