@@ -47,6 +47,20 @@ __device__ __forceinline__ cd crecip(cd b) {
         return cmk(r / d, -1.0 / d);
     }
 }
+// 1/b via an exact power-of-two scaling, v_rcp_f64 and two Newton steps (about 1 ulp,
+// no IEEE division sequences on the pivot's critical path).
+__device__ __forceinline__ cd crecip_fast(cd b) {
+    const double m = fmax(fabs(b.x), fabs(b.y));
+    if (!(m > 0.0) || !(m < INFINITY)) return crecip(b);   // 0, inf, nan: reference path
+    const int e = __builtin_amdgcn_frexp_exp(m);
+    const double bx = __builtin_amdgcn_ldexp(b.x, -e), by = __builtin_amdgcn_ldexp(b.y, -e);
+    const double d = fma(bx, bx, by * by);                  // in [1/4, 2)
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return cmk(__builtin_amdgcn_ldexp(bx * r, -e), __builtin_amdgcn_ldexp(-by * r, -e));
+}
+
 // |re| + |im|, the pivot measure of LAPACK izamax (dcabs1).
 __device__ __forceinline__ double cabs1(cd a) { return fabs(a.x) + fabs(a.y); }
 
@@ -702,17 +716,17 @@ __device__ __forceinline__ void gje3_step(cd (&P)[PB / 2], const int J, const in
         for (int q = 0; q < HB; ++q) s.rowC[par][h * HB + q] = P[q];
     }
     __syncthreads();
-    double bv = s.rv[par][0];
-    int p = s.ri[par][0], wb = 0;
-    for (int i = 1; i < nw; ++i) {
-        const double ov = s.rv[par][i];
-        const int oi = s.ri[par][i];
-        if (ov > bv || (ov == bv && oi < p)) { bv = ov; p = oi; wb = i; }
-    }
+    // winner over the waves: lane i reads wave i's candidate; the first lane holding the
+    // maximum is the lowest wave, hence the lowest row among equal values
+    const double wv = lane < nw ? s.rv[par][lane] : -2.0;
+    const int wi = lane < nw ? s.ri[par][lane] : 0x7fffffff;
+    const double bmax = wave_max(wv);
+    const int wb = __ffsll((long long)__ballot(wv == bmax)) - 1;
+    int p = __builtin_amdgcn_readlane(wi, wb);
     if (p < c || p >= Np) p = c;
     const cd *prow = (p == c) ? s.rowC[par] : s.cand[par][wb];
     const cd pv = prow[J];
-    const cd inv = crecip(pv);
+    const cd inv = crecip_fast(pv);
     if (t == 0) {
         s.ipiv[c] = p;
         if (p != c) {
