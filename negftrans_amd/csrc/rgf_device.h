@@ -392,6 +392,7 @@ __device__ __forceinline__ void copy_cols(cd *__restrict__ dst, const cd *__rest
 #endif
 #define NEGF_CB2 4    // columns per streamed block
 #define NEGF_MAXW 16  // waves per workgroup (1024 threads)
+#define NEGF_TCH 128  // columns of the old pivot rows staged in LDS at a time
 
 template <int PB>
 struct GjeShared {
@@ -673,7 +674,7 @@ struct Gje3Shared {
     int ri[2][NEGF_MAXW];
     cd cand[2][NEGF_MAXW][PB];
     cd rowC[2][PB];
-    cd Tw[NEGF_MAXW][PB * NEGF_CB2];
+    cd Tl[PB * NEGF_TCH];      // old pivot rows of one column chunk: Tl[j + (c - c0) PB]
     double red[NEGF_MAXW];
 };
 
@@ -768,7 +769,7 @@ __device__ __forceinline__ int gje_invert3(cd *__restrict__ Z, const int Np, Gje
                                            cd *__restrict__ scratch) {
     constexpr int HB = PB / 2;
     constexpr int CB = NEGF_CB2;
-    static_assert(CB % 2 == 0 && PB * CB == 128, "block shape: two T entries per lane");
+    static_assert(CB % 2 == 0 && NEGF_TCH % PB == 0, "block shape");
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int r = t >> 1, h = t & 1;
     const bool own = r < Np;
@@ -825,57 +826,62 @@ __device__ __forceinline__ int gje_invert3(cd *__restrict__ Z, const int Np, Gje
             Z[s.disp[q] + col * ld] = Dg[q + (size_t)col * PB];
         }
         __syncthreads();
-        // streamed rank-PB update: lane h accumulates its half of the panel columns for
-        // the CB columns of a block, the partner's partial sums arrive through a lane
-        // swap, and lane h stores the block columns cc with cc % 2 == h.
+        // streamed rank-PB update, one chunk of NEGF_TCH columns at a time: the chunk's
+        // old pivot rows are staged in LDS by the whole workgroup, then every wave streams
+        // its rows through the chunk's column blocks with a two-block register prefetch.
+        // Lane h accumulates its half of the panel columns, the partner's partial sums
+        // arrive through a lane swap, and lane h stores the block columns cc % 2 == h.
         const bool pivrow = (r >= k0 && r < k0 + PB);
         const bool ld_ok = own && !pivrow;
-        cd *tw = s.Tw[w];
-        int cb = (k0 == 0) ? PB : 0;
-        // lane h owns block columns h, h+2, ...: a[i] is the old value of column 2i+h
-        // (the two staged T entries of a lane are named variables: as an array the
-        // compiler placed them in scratch, which serialised every prefetch)
-        cd a[CB / 2], tn0 = cmk(0.0, 0.0), tn1 = cmk(0.0, 0.0);
-        if (cb < Np) {
-#pragma unroll
-            for (int i = 0; i < CB / 2; ++i) a[i] = ld_ok ? Z[r + (cb + 2 * i + h) * ld] : cmk(0.0, 0.0);
-            tn0 = Tg[lane + (size_t)cb * PB];
-            tn1 = Tg[lane + 64 + (size_t)cb * PB];
-        }
-        while (cb < Np) {
-            int cbn = cb + CB;
-            if (cbn >= k0 && cbn < k0 + PB) cbn = k0 + PB;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            tw[lane] = tn0;
-            tw[lane + 64] = tn1;
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            cd an[CB / 2];
-            if (cbn < Np) {
-#pragma unroll
-                for (int i = 0; i < CB / 2; ++i) an[i] = ld_ok ? Z[r + (cbn + 2 * i + h) * ld] : cmk(0.0, 0.0);
-                tn0 = Tg[lane + (size_t)cbn * PB];
-                tn1 = Tg[lane + 64 + (size_t)cbn * PB];
+        for (int c0 = 0; c0 < Np; c0 += NEGF_TCH) {
+            const int c1 = min(c0 + NEGF_TCH, Np);
+            if (c0 >= k0 && c1 <= k0 + PB) continue;       // chunk is all panel
+            for (int e = t; e < (c1 - c0) * PB; e += blockDim.x) {
+                const int col = c0 + e / PB;
+                if (!(col >= k0 && col < k0 + PB)) s.Tl[e] = Tg[e + (size_t)c0 * PB];
             }
-            // four partial sums per column (j mod 4) are independent FMA chains, so
-            // consecutive FMAs do not wait on each other
+            __syncthreads();
+            auto nxt = [&](int cb) {
+                cb += CB;
+                if (cb >= k0 && cb < k0 + PB) cb = k0 + PB;
+                return cb;
+            };
+            int cb = (c0 >= k0 && c0 < k0 + PB) ? k0 + PB : c0;
+            int cb1 = nxt(cb), cb2 = nxt(cb1);
+            cd a0[CB / 2], a1[CB / 2];
 #pragma unroll
-            for (int cc = 0; cc < CB; ++cc) {
-                const cd *tcol = tw + cc * PB + h * HB;
-                cd p4[4] = {cmk(0.0, 0.0), cmk(0.0, 0.0), cmk(0.0, 0.0), cmk(0.0, 0.0)};
-#pragma unroll
-                for (int j = 0; j < HB; ++j) p4[j & 3] = cfma(P[j], tcol[j], p4[j & 3]);
-                const cd sacc = cadd(cadd(p4[0], p4[1]), cadd(p4[2], p4[3]));
-                // a[cc >> 1] is this lane's base only when cc % 2 == h, the one case stored
-                const cd tot = cadd(cadd(a[cc >> 1], sacc), lane_swap(sacc));
-                if (own && (cc & 1) == h) Z[r + (cb + cc) * ld] = tot;
+            for (int i = 0; i < CB / 2; ++i) {
+                a0[i] = (ld_ok && cb < c1) ? Z[r + (cb + 2 * i + h) * ld] : cmk(0.0, 0.0);
+                a1[i] = (ld_ok && cb1 < c1) ? Z[r + (cb1 + 2 * i + h) * ld] : cmk(0.0, 0.0);
             }
+            while (cb < c1) {
+                cd a2[CB / 2];
 #pragma unroll
-            for (int i = 0; i < CB / 2; ++i) a[i] = an[i];
-            cb = cbn;
+                for (int i = 0; i < CB / 2; ++i)
+                    a2[i] = (ld_ok && cb2 < c1) ? Z[r + (cb2 + 2 * i + h) * ld] : cmk(0.0, 0.0);
+                // four partial sums per column (j mod 4) are independent FMA chains
+#pragma unroll
+                for (int cc = 0; cc < CB; ++cc) {
+                    const cd *tcol = s.Tl + (cb + cc - c0) * PB + h * HB;
+                    cd p4[4] = {cmk(0.0, 0.0), cmk(0.0, 0.0), cmk(0.0, 0.0), cmk(0.0, 0.0)};
+#pragma unroll
+                    for (int j = 0; j < HB; ++j) p4[j & 3] = cfma(P[j], tcol[j], p4[j & 3]);
+                    const cd sacc = cadd(cadd(p4[0], p4[1]), cadd(p4[2], p4[3]));
+                    // a0[cc >> 1] is this lane's base only when cc % 2 == h, the one case stored
+                    const cd tot = cadd(cadd(a0[cc >> 1], sacc), lane_swap(sacc));
+                    if (own && (cc & 1) == h) Z[r + (cb + cc) * ld] = tot;
+                }
+#pragma unroll
+                for (int i = 0; i < CB / 2; ++i) {
+                    a0[i] = a1[i];
+                    a1[i] = a2[i];
+                }
+                cb = cb1;
+                cb1 = cb2;
+                cb2 = nxt(cb2);
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (t == 0) {
         for (int c = 0; c < Np; ++c) s.sigma[c] = c;
