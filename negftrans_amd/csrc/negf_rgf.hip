@@ -104,13 +104,14 @@ __device__ __forceinline__ void row_product(cd *__restrict__ Rn, const cd *__res
     __syncthreads();
 }
 
-// TWO: two threads per matrix row (gje_invert3); otherwise one (gje_invert2).
+// TWO: two lanes per matrix row and the trailing update on the FP64 matrix cores
+// (gje_invert4, Np <= 512); otherwise one thread per row (gje_invert2).
 template <int PB, bool TWO>
-using GjeSh = typename std::conditional<TWO, Gje3Shared<PB>, GjeShared<PB>>::type;
+using GjeSh = typename std::conditional<TWO, Gje4Shared<PB>, GjeShared<PB>>::type;
 
 template <int PB, bool TWO>
 __device__ __forceinline__ int gje_any(cd *Z, int Np, GjeSh<PB, TWO> &s, cd *scratch) {
-    if constexpr (TWO) return gje_invert3<PB>(Z, Np, s, scratch);
+    if constexpr (TWO) return gje_invert4<PB>(Z, Np, s, scratch);
     else return gje_invert2<PB>(Z, Np, s, scratch);
 }
 

@@ -895,3 +895,304 @@ __device__ __forceinline__ int gje_invert3(cd *__restrict__ Z, const int Np, Gje
     __syncthreads();
     return s.flag;
 }
+
+// ====================================================================================
+// Gauss-Jordan inversion with the trailing update on the FP64 matrix cores.
+//
+// Np % 32 == 0, Np <= NEGF_MAXNP4, blockDim.x == 2 Np.  Wave w owns rows [32 w, 32 w + 32)
+// as two 16-row tiles.  Lane l = j + 16 k (j = l & 15, k = l >> 4) holds, for tile t,
+// the panel values P[t][q] = Z[32 w + 16 t + j][k0 + 4 q + k] -- exactly the B operand
+// of v_mfma_f64_16x16x4f64 (B[k][j] at lane j + 16 k) for the products
+//     Z[rows][ct .. ct+15] += P[rows][4q .. 4q+3] * T[4q .. 4q+3][ct .. ct+15]
+// computed as D = A B with A[i][k] = T[4q + k][ct + i] (lane i + 16 k) and D[i][j] in
+// lane j + 16 (i % 4), register i / 4.  T (the old pivot rows) is staged in LDS.
+// During the pivot steps the panel registers rotate once per four steps, so the current
+// column always sits in P[t][0] (no runtime register indexing).
+// ====================================================================================
+#define NEGF_MAXNP4 512
+#define NEGF_LDT (NEGF_TCH + 1)   // LDS row stride of the staged pivot rows (padded)
+
+typedef double negf_d4 __attribute__((ext_vector_type(4)));
+
+template <int PB>
+struct Gje4Shared {
+    int perm[NEGF_MAXNP4];
+    int ipiv[NEGF_MAXNP4];
+    int sigma[NEGF_MAXNP4];
+    int disp[2 * PB];
+    int ndisp;
+    int flag;
+    double rv[2][NEGF_MAXW];
+    int ri[2][NEGF_MAXW];
+    cd cand[2][NEGF_MAXW][PB];   // each wave's best candidate row (panel columns)
+    cd rowC[2][PB];              // row c (the pivot position) before the swap
+    cd colJ[2][NEGF_MAXNP4];     // panel column J of every row
+    cd Tl[PB * NEGF_LDT];        // old pivot rows of one column chunk: Tl[p * LDT + (c - c0)]
+    double red[NEGF_MAXW];
+};
+
+// Max over the 16 lanes of DPP row `row`, returned wave-uniform.
+__device__ __forceinline__ double row16_max(double v, int row) {
+    v = dpp_max_step<0xB1, 0xf>(v);
+    v = dpp_max_step<0x4E, 0xf>(v);
+    v = dpp_max_step<0x141, 0xf>(v);
+    v = dpp_max_step<0x140, 0xf>(v);
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), row << 4);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), row << 4);
+    return __hiloint2double(hi, lo);
+}
+
+// Elimination of panel column J from one row's panel values (this lane's share).
+template <int KK>
+__device__ __forceinline__ void gje4_row_update(cd (&P)[KK], const int r, cd mr, const int c, const int p,
+                                                const int J, const int g, const int k, const bool mine,
+                                                const cd inv, const cd (&pr)[KK], const cd *rowC) {
+    const bool piv = (r == c), takeC = (r == p && r != c);
+    if (takeC) {
+        mr = rowC[J];
+#pragma unroll
+        for (int q = 0; q < KK; ++q) P[q] = rowC[4 * ((q + g) % KK) + k];
+    }
+    if (piv) {
+#pragma unroll
+        for (int q = 0; q < KK; ++q) P[q] = cmk(0.0, 0.0);
+    }
+    const cd f = piv ? cmk(-inv.x, -inv.y) : cmul(mr, inv);
+#pragma unroll
+    for (int q = 0; q < KK; ++q) {
+        const cd nv = cfms(f, pr[q], P[q]);
+        P[q] = (q == 0 && mine) ? cmk(-f.x, -f.y) : nv;
+    }
+}
+
+// One pivot step, column J = 4 g + KJ of the panel; P[.][q] holds panel column
+// 4 ((q + g) % (PB / 4)) + k.
+template <int PB, int KJ>
+__device__ __forceinline__ void gje4_step(cd (&P)[2][PB / 4], const int g, const int k0, const int Np,
+                                          const int w, const int j, const int k, Gje4Shared<PB> &s) {
+    constexpr int KK = PB / 4;
+    const int t = threadIdx.x, lane = t & 63, nw = blockDim.x >> 6;
+    const int J = 4 * g + KJ, c = k0 + J, par = J & 1;
+    const int r0 = (w << 5) + j, r1 = r0 + 16;
+    const bool mine = (k == KJ);
+    double v0 = -1.0, v1 = -1.0;
+    if (mine) {
+        if (r0 >= c) {
+            v0 = cabs1(P[0][0]);
+            if (!(v0 >= 0.0)) v0 = -0.5;
+        }
+        if (r1 >= c) {
+            v1 = cabs1(P[1][0]);
+            if (!(v1 >= 0.0)) v1 = -0.5;
+        }
+        s.colJ[par][r0] = P[0][0];
+        s.colJ[par][r1] = P[1][0];
+    }
+    // izamax order: largest |re| + |im|, lowest row on ties (tile 0 rows precede tile 1)
+    const double vmax = row16_max(v0 > v1 ? v0 : v1, KJ);
+    int idx = 0x7fffffff;
+    if (vmax > -1.0) {
+        const unsigned long long h0 = __ballot(v0 == vmax), h1 = __ballot(v1 == vmax);
+        idx = h0 ? (w << 5) + ((__ffsll((long long)h0) - 1) & 15)
+                 : (w << 5) + 16 + ((__ffsll((long long)h1) - 1) & 15);
+    }
+    if (lane == 0) {
+        s.rv[par][w] = vmax;
+        s.ri[par][w] = idx;
+    }
+    // (separate stores per tile: a select between P[0] and P[1] would become a select
+    // of addresses and push P out of registers)
+    const bool cnd = idx != 0x7fffffff && j == (idx & 15);
+    const bool rc = (c >> 5) == w && j == (c & 15);
+    if (cnd && !((idx >> 4) & 1)) {
+#pragma unroll
+        for (int q = 0; q < KK; ++q) s.cand[par][w][4 * ((q + g) % KK) + k] = P[0][q];
+    }
+    if (cnd && ((idx >> 4) & 1)) {
+#pragma unroll
+        for (int q = 0; q < KK; ++q) s.cand[par][w][4 * ((q + g) % KK) + k] = P[1][q];
+    }
+    if (rc && !((c >> 4) & 1)) {
+#pragma unroll
+        for (int q = 0; q < KK; ++q) s.rowC[par][4 * ((q + g) % KK) + k] = P[0][q];
+    }
+    if (rc && ((c >> 4) & 1)) {
+#pragma unroll
+        for (int q = 0; q < KK; ++q) s.rowC[par][4 * ((q + g) % KK) + k] = P[1][q];
+    }
+    __syncthreads();
+    // winner over the waves (at most 16, all in DPP row 0): first wave holding the max
+    const double wv = lane < nw ? s.rv[par][lane] : -2.0;
+    const int wi = lane < nw ? s.ri[par][lane] : 0x7fffffff;
+    const double bmax = row16_max(wv, 0);
+    const int wb = __ffsll((long long)__ballot(wv == bmax)) - 1;
+    int p = __builtin_amdgcn_readlane(wi, wb);
+    if (p < c || p >= Np) p = c;
+    const cd *prow = (p == c) ? s.rowC[par] : s.cand[par][wb];
+    const cd inv = crecip_fast(prow[J]);
+    if (t == 0) {
+        s.ipiv[c] = p;
+        if (p != c) {
+            const int tt = s.perm[c];
+            s.perm[c] = s.perm[p];
+            s.perm[p] = tt;
+        }
+        if (prow[J].x == 0.0 && prow[J].y == 0.0 && s.flag == 0) s.flag = c + 1;
+    }
+    cd pr[KK];
+#pragma unroll
+    for (int q = 0; q < KK; ++q) pr[q] = prow[4 * ((q + g) % KK) + k];
+    const cd m0 = s.colJ[par][r0], m1 = s.colJ[par][r1];
+    gje4_row_update<KK>(P[0], r0, m0, c, p, J, g, k, mine, inv, pr, s.rowC[par]);
+    gje4_row_update<KK>(P[1], r1, m1, c, p, J, g, k, mine, inv, pr, s.rowC[par]);
+}
+
+// In-place inversion of Z (Np x Np) by a workgroup of 2 Np threads; on return
+// inverse[:, c] = Z[:, s.sigma[c]].  scratch: (2 PB) x Np complex per workgroup.  Returns
+// the 1-based column of the first exactly-zero pivot, or 0.
+template <int PB>
+__device__ __forceinline__ int gje_invert4(cd *__restrict__ Z, const int Np, Gje4Shared<PB> &s,
+                                           cd *__restrict__ scratch) {
+    static_assert(PB == 32, "two 16-row tiles per wave hold one 32-column panel");
+    constexpr int KK = PB / 4;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int j = lane & 15, k = lane >> 4;
+    const int R0 = w << 5;
+    const size_t ld = (size_t)Np;
+    cd *Tg = scratch;                      // old pivot rows: Tg[p + c PB]
+    cd *Dg = scratch + (size_t)PB * Np;    // displaced rows: Dg[d + c PB]
+    if (t == 0) s.flag = 0;
+    for (int k0 = 0; k0 < Np; k0 += PB) {
+        cd P[2][KK];
+#pragma unroll
+        for (int q = 0; q < KK; ++q) {
+            const cd *zc = Z + (R0 + j) + (size_t)(k0 + 4 * q + k) * ld;
+            P[0][q] = zc[0];
+            P[1][q] = zc[16];
+        }
+        for (int i = t; i < Np; i += blockDim.x) s.perm[i] = i;
+        if (t == 0) s.ndisp = 0;
+        __syncthreads();
+#pragma unroll 1
+        for (int g = 0; g < KK; ++g) {
+            gje4_step<PB, 0>(P, g, k0, Np, w, j, k, s);
+            gje4_step<PB, 1>(P, g, k0, Np, w, j, k, s);
+            gje4_step<PB, 2>(P, g, k0, Np, w, j, k, s);
+            gje4_step<PB, 3>(P, g, k0, Np, w, j, k, s);
+            const cd x0 = P[0][0], x1 = P[1][0];
+#pragma unroll
+            for (int q = 0; q < KK - 1; ++q) {
+                P[0][q] = P[0][q + 1];
+                P[1][q] = P[1][q + 1];
+            }
+            P[0][KK - 1] = x0;
+            P[1][KK - 1] = x1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < KK; ++q) {
+            cd *zc = Z + (R0 + j) + (size_t)(k0 + 4 * q + k) * ld;
+            zc[0] = P[0][q];
+            zc[16] = P[1][q];
+        }
+        if (k == 0) {
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt) {
+                const int r = R0 + 16 * tt + j;
+                if (r >= k0 + PB && s.perm[r] != r) s.disp[atomicAdd(&s.ndisp, 1)] = r;
+            }
+        }
+        __syncthreads();
+        const int nd = s.ndisp;
+        const int rows = PB + nd;
+        {
+            constexpr int GB = 4;
+            const int total = rows * Np;
+            for (int e0 = t; e0 < total; e0 += GB * blockDim.x) {
+                cd gv[GB];
+#pragma unroll
+                for (int gg = 0; gg < GB; ++gg) {
+                    const int e = e0 + gg * blockDim.x;
+                    const int q = e % rows, col = e / rows;
+                    const bool ok = e < total && !(col >= k0 && col < k0 + PB);
+                    const int src = ok ? (q < PB ? s.perm[k0 + q] : s.perm[s.disp[q - PB]]) : 0;
+                    gv[gg] = ok ? Z[src + col * ld] : cmk(0.0, 0.0);
+                }
+#pragma unroll
+                for (int gg = 0; gg < GB; ++gg) {
+                    const int e = e0 + gg * blockDim.x;
+                    const int q = e % rows, col = e / rows;
+                    if (e < total && !(col >= k0 && col < k0 + PB)) {
+                        if (q < PB) Tg[q + (size_t)col * PB] = gv[gg];
+                        else Dg[(q - PB) + (size_t)col * PB] = gv[gg];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int e = t; e < nd * Np; e += blockDim.x) {
+            const int q = e % nd, col = e / nd;
+            if (col >= k0 && col < k0 + PB) continue;
+            Z[s.disp[q] + col * ld] = Dg[q + (size_t)col * PB];
+        }
+        __syncthreads();
+        // trailing update on the matrix cores, one chunk of NEGF_TCH columns at a time;
+        // the pivot rows themselves (this wave's rows when R0 == k0) start from zero
+        const bool zb = (R0 == k0);
+        for (int c0 = 0; c0 < Np; c0 += NEGF_TCH) {
+            const int c1 = min(c0 + NEGF_TCH, Np);
+            if (c0 >= k0 && c1 <= k0 + PB) continue;       // chunk is all panel
+            for (int e = t; e < (c1 - c0) * PB; e += blockDim.x) {
+                const int pp = e % PB, cl = e / PB;
+                const int col = c0 + cl;
+                if (!(col >= k0 && col < k0 + PB)) s.Tl[pp * NEGF_LDT + cl] = Tg[e + (size_t)c0 * PB];
+            }
+            __syncthreads();
+            for (int ct = c0; ct < c1; ct += 16) {
+                if (ct >= k0 && ct < k0 + PB) continue;
+                cd *zt = Z + (R0 + j) + (size_t)(ct + k) * ld;
+                negf_d4 ar0, ai0, ar1, ai1;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const cd b0 = zb ? cmk(0.0, 0.0) : zt[(size_t)4 * q * ld];
+                    const cd b1 = zb ? cmk(0.0, 0.0) : zt[(size_t)4 * q * ld + 16];
+                    ar0[q] = b0.x;
+                    ai0[q] = b0.y;
+                    ar1[q] = b1.x;
+                    ai1[q] = b1.y;
+                }
+                const cd *tl = s.Tl + k * NEGF_LDT + (ct - c0 + j);
+#pragma unroll
+                for (int q = 0; q < KK; ++q) {
+                    const cd tv = tl[4 * q * NEGF_LDT];
+                    const double ntim = -tv.y;
+                    ar0 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, P[0][q].x, ar0, 0, 0, 0);
+                    ar1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, P[1][q].x, ar1, 0, 0, 0);
+                    ai0 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.y, P[0][q].x, ai0, 0, 0, 0);
+                    ai1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.y, P[1][q].x, ai1, 0, 0, 0);
+                    ar0 = __builtin_amdgcn_mfma_f64_16x16x4f64(ntim, P[0][q].y, ar0, 0, 0, 0);
+                    ar1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ntim, P[1][q].y, ar1, 0, 0, 0);
+                    ai0 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, P[0][q].y, ai0, 0, 0, 0);
+                    ai1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, P[1][q].y, ai1, 0, 0, 0);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    zt[(size_t)4 * q * ld] = cmk(ar0[q], ai0[q]);
+                    zt[(size_t)4 * q * ld + 16] = cmk(ar1[q], ai1[q]);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (t == 0) {
+        for (int c = 0; c < Np; ++c) s.sigma[c] = c;
+        for (int c = Np - 1; c >= 0; --c) {
+            const int p = s.ipiv[c];
+            const int tt = s.sigma[c];
+            s.sigma[c] = s.sigma[p];
+            s.sigma[p] = tt;
+        }
+    }
+    __syncthreads();
+    return s.flag;
+}
