@@ -913,6 +913,25 @@ __device__ __forceinline__ int gje_invert3(cd *__restrict__ Z, const int Np, Gje
 #define NEGF_LDT (NEGF_TCH + 1)   // LDS row stride of the staged pivot rows (padded)
 
 typedef double negf_d4 __attribute__((ext_vector_type(4)));
+typedef int negf_i4 __attribute__((ext_vector_type(4)));
+
+// Raw buffer access to a complex128 array: one 32-bit VGPR offset per lane plus a
+// wave-uniform SGPR offset, instead of a 64-bit address per access.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t negf_rsrc(cd *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ cd buf_load_cd(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    const negf_i4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    return cmk(__hiloint2double(v.y, v.x), __hiloint2double(v.w, v.z));
+}
+__device__ __forceinline__ void buf_store_cd(cd x, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    negf_i4 v;
+    v.x = __double2loint(x.x);
+    v.y = __double2hiint(x.x);
+    v.z = __double2loint(x.y);
+    v.w = __double2hiint(x.y);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
+}
 
 template <int PB>
 struct Gje4Shared {
@@ -940,29 +959,6 @@ __device__ __forceinline__ double row16_max(double v, int row) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), row << 4);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), row << 4);
     return __hiloint2double(hi, lo);
-}
-
-// Elimination of panel column J from one row's panel values (this lane's share).
-template <int KK>
-__device__ __forceinline__ void gje4_row_update(cd (&P)[KK], const int r, cd mr, const int c, const int p,
-                                                const int J, const int g, const int k, const bool mine,
-                                                const cd inv, const cd (&pr)[KK], const cd *rowC) {
-    const bool piv = (r == c), takeC = (r == p && r != c);
-    if (takeC) {
-        mr = rowC[J];
-#pragma unroll
-        for (int q = 0; q < KK; ++q) P[q] = rowC[4 * ((q + g) % KK) + k];
-    }
-    if (piv) {
-#pragma unroll
-        for (int q = 0; q < KK; ++q) P[q] = cmk(0.0, 0.0);
-    }
-    const cd f = piv ? cmk(-inv.x, -inv.y) : cmul(mr, inv);
-#pragma unroll
-    for (int q = 0; q < KK; ++q) {
-        const cd nv = cfms(f, pr[q], P[q]);
-        P[q] = (q == 0 && mine) ? cmk(-f.x, -f.y) : nv;
-    }
 }
 
 // One pivot step, column J = 4 g + KJ of the panel; P[.][q] holds panel column
@@ -1039,12 +1035,37 @@ __device__ __forceinline__ void gje4_step(cd (&P)[2][PB / 4], const int g, const
         }
         if (prow[J].x == 0.0 && prow[J].y == 0.0 && s.flag == 0) s.flag = c + 1;
     }
-    cd pr[KK];
+    // eliminate column J from both rows of this lane: the pivot row position takes the
+    // scaled pivot row, the row that supplied the pivot takes old row c
+    cd m0 = s.colJ[par][r0], m1 = s.colJ[par][r1];
+    const bool piv0 = (r0 == c), piv1 = (r1 == c);
+    if (r0 == p && p != c) {
+        m0 = s.rowC[par][J];
 #pragma unroll
-    for (int q = 0; q < KK; ++q) pr[q] = prow[4 * ((q + g) % KK) + k];
-    const cd m0 = s.colJ[par][r0], m1 = s.colJ[par][r1];
-    gje4_row_update<KK>(P[0], r0, m0, c, p, J, g, k, mine, inv, pr, s.rowC[par]);
-    gje4_row_update<KK>(P[1], r1, m1, c, p, J, g, k, mine, inv, pr, s.rowC[par]);
+        for (int q = 0; q < KK; ++q) P[0][q] = s.rowC[par][4 * ((q + g) % KK) + k];
+    }
+    if (r1 == p && p != c) {
+        m1 = s.rowC[par][J];
+#pragma unroll
+        for (int q = 0; q < KK; ++q) P[1][q] = s.rowC[par][4 * ((q + g) % KK) + k];
+    }
+    if (piv0) {
+#pragma unroll
+        for (int q = 0; q < KK; ++q) P[0][q] = cmk(0.0, 0.0);
+    }
+    if (piv1) {
+#pragma unroll
+        for (int q = 0; q < KK; ++q) P[1][q] = cmk(0.0, 0.0);
+    }
+    const cd f0 = piv0 ? cmk(-inv.x, -inv.y) : cmul(m0, inv);
+    const cd f1 = piv1 ? cmk(-inv.x, -inv.y) : cmul(m1, inv);
+#pragma unroll
+    for (int q = 0; q < KK; ++q) {
+        const cd pq = prow[4 * ((q + g) % KK) + k];
+        const cd n0 = cfms(f0, pq, P[0][q]), n1 = cfms(f1, pq, P[1][q]);
+        P[0][q] = (q == 0 && mine) ? cmk(-f0.x, -f0.y) : n0;
+        P[1][q] = (q == 0 && mine) ? cmk(-f1.x, -f1.y) : n1;
+    }
 }
 
 // In-place inversion of Z (Np x Np) by a workgroup of 2 Np threads; on return
@@ -1137,8 +1158,27 @@ __device__ __forceinline__ int gje_invert4(cd *__restrict__ Z, const int Np, Gje
         }
         __syncthreads();
         // trailing update on the matrix cores, one chunk of NEGF_TCH columns at a time;
-        // the pivot rows themselves (this wave's rows when R0 == k0) start from zero
+        // the pivot rows themselves (this wave's rows when R0 == k0) start from zero.
+        // Column tiles run in increasing order (panel tiles skipped) and the base values
+        // of the next tile are loaded while the current one is on the matrix cores.
         const bool zb = (R0 == k0);
+        const __amdgpu_buffer_rsrc_t zr = negf_rsrc(Z, (unsigned)(Np * Np) * 16u);
+        const unsigned cstr = 64u * (unsigned)Np;          // bytes from column c to c + 4
+        auto next_tile = [&](int ct) {                      // last tile: reload itself (unused)
+            const int n = (ct + 16 == k0) ? k0 + PB : ct + 16;
+            return n < Np ? n : ct;
+        };
+        auto voff = [&](int ct) { return (unsigned)((R0 + j) + (ct + k) * Np) * 16u; };
+        cd nb0[4], nb1[4];
+        auto load_base = [&](int ct) {
+            const unsigned vo = voff(ct);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                nb0[q] = buf_load_cd(zr, vo, q * cstr);
+                nb1[q] = buf_load_cd(zr, vo + 256u, q * cstr);
+            }
+        };
+        load_base(k0 == 0 ? PB : 0);
         for (int c0 = 0; c0 < Np; c0 += NEGF_TCH) {
             const int c1 = min(c0 + NEGF_TCH, Np);
             if (c0 >= k0 && c1 <= k0 + PB) continue;       // chunk is all panel
@@ -1150,18 +1190,19 @@ __device__ __forceinline__ int gje_invert4(cd *__restrict__ Z, const int Np, Gje
             __syncthreads();
             for (int ct = c0; ct < c1; ct += 16) {
                 if (ct >= k0 && ct < k0 + PB) continue;
-                cd *zt = Z + (R0 + j) + (size_t)(ct + k) * ld;
                 negf_d4 ar0, ai0, ar1, ai1;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const cd b0 = zb ? cmk(0.0, 0.0) : zt[(size_t)4 * q * ld];
-                    const cd b1 = zb ? cmk(0.0, 0.0) : zt[(size_t)4 * q * ld + 16];
-                    ar0[q] = b0.x;
-                    ai0[q] = b0.y;
-                    ar1[q] = b1.x;
-                    ai1[q] = b1.y;
+                    ar0[q] = zb ? 0.0 : nb0[q].x;
+                    ai0[q] = zb ? 0.0 : nb0[q].y;
+                    ar1[q] = zb ? 0.0 : nb1[q].x;
+                    ai1[q] = zb ? 0.0 : nb1[q].y;
                 }
-                const cd *tl = s.Tl + k * NEGF_LDT + (ct - c0 + j);
+                load_base(next_tile(ct));
+                __builtin_amdgcn_sched_barrier(0);          // keep the prefetch ahead of the MFMAs
+                int lx = threadIdx.x & 63;
+                asm volatile("" : "+v"(lx));                 // recompute, do not keep live
+                const cd *tl = s.Tl + (lx >> 4) * NEGF_LDT + (ct - c0 + (lx & 15));
 #pragma unroll
                 for (int q = 0; q < KK; ++q) {
                     const cd tv = tl[4 * q * NEGF_LDT];
@@ -1175,10 +1216,11 @@ __device__ __forceinline__ int gje_invert4(cd *__restrict__ Z, const int Np, Gje
                     ai0 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, P[0][q].y, ai0, 0, 0, 0);
                     ai1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, P[1][q].y, ai1, 0, 0, 0);
                 }
+                const unsigned vo = voff(ct);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    zt[(size_t)4 * q * ld] = cmk(ar0[q], ai0[q]);
-                    zt[(size_t)4 * q * ld + 16] = cmk(ar1[q], ai1[q]);
+                    buf_store_cd(cmk(ar0[q], ai0[q]), zr, vo, q * cstr);
+                    buf_store_cd(cmk(ar1[q], ai1[q]), zr, vo + 256u, q * cstr);
                 }
             }
             __syncthreads();
