@@ -25,7 +25,7 @@ typedef struct negf_ctx negf_ctx;
 
 /* Precision of the bulk block products (flags of negf_ctx_create). */
 #define NEGF_PREC_FP64       0   /* complex128 everywhere (parity path)              */
-#define NEGF_PREC_MIXED_FP32 1   /* FP32 bulk, FP64 pivots/diagonal, FP64 refinement */
+#define NEGF_PREC_MIXED_FP32 1   /* FP32 sweep kept in HBM + FP64 iterative refinement  */
 /* V-hat window: faithful copies only |p|<=NGX/2, |q|<=NGY/2 (src/main.f90:242-246). */
 #define NEGF_FULL_WINDOW     16  /* extension: use the whole +-NGX, +-NGY window    */
 
@@ -156,6 +156,12 @@ int negf_transmission_trace(int N, int N_z, const double *G, double E, const int
 
 /* Wall time (seconds) of the last batched call's device work, from HIP events. */
 double negf_last_kernel_seconds(negf_ctx *ctx);
+
+/* Extension, no reference counterpart.  Number of energies of the last
+ * negf_transmission_batch call on a NEGF_PREC_MIXED_FP32 context whose refinement did
+ * not reach its tolerance within the iteration limit and were recomputed on the FP64
+ * path (their results are FP64 results).  0 for FP64 contexts. */
+int negf_last_fallbacks(const negf_ctx *ctx);
 
 #ifdef __cplusplus
 }

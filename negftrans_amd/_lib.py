@@ -38,6 +38,7 @@ SIGNATURES = {
     "negf_inverse_batch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "negf_matrix_product_batch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
     "negf_last_kernel_seconds": (ctypes.c_double, [_vp]),
+    "negf_last_fallbacks": (ctypes.c_int, [_vp]),
     "negf_hamiltonian_direct": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
                                                ctypes.c_int, ctypes.c_int, _dp]),

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -177,6 +178,300 @@ __global__ __launch_bounds__(MAXT) void k_transmission(TransArgs a) {
         if (threadIdx.x == 0) {
             a.tau[job] = tot * sv.scale * sv.scale;
             a.status[job] = bad;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ mixed precision
+// NEGF_PREC_MIXED_FP32: the backward sweep runs in FP32 (build rounded from FP64, FP32
+// Gauss-Jordan on the matrix cores) and keeps every generator g_m (and its column
+// permutation) in HBM -- N_z Np^2 complex64 per workgroup, 0.82 GB at Np = 320,
+// N_z = 1000.  G(:, 1) on the open columns then solves A X = E_1(:, open) for the scaled
+// block-tridiagonal A = tridiag(I, D_m, I) of GreensFunction_tri_solver
+// (src/math_kernel.f90:44-165): X_1 = g_1 E_1, X_m = -g_m X_{m-1}, followed by
+// iterative refinement with FP64 residuals r = E_1 - A X (D_m rebuilt in FP64 from the
+// V-hat table) and correction solves through the stored generators,
+//     y_m = g_m (r_m - y_{m+1})  (m = N_z .. 1),  d_1 = y_1,  d_m = y_m - g_m d_{m-1},
+// X += d.  The trace uses X_{N_z} = G_{N_z,1}(:, open).
+#define NEGF_NRH 16   // right-hand-side columns of one solve (open channels at slice 1)
+
+struct MixedArgs {
+    TransArgs a;      // job tables (a.ws unused)
+    int nref;         // refinement iterations
+    int ldb;          // columns of X and Y (max n_R over the jobs, <= NEGF_NRH)
+    cfl *hist;        // per slot N_z Np^2: g_m, column-permuted like the Gauss-Jordan output
+    int *hsig;        // per slot N_z Np: their column permutations
+    cfl *scr;         // per slot 2 * 32 * Np: Gauss-Jordan scratch
+    cd *X;            // per slot N_z Np ldb: G(:, 1)(:, open), row-major per slice
+    cfl *Y;           // per slot N_z Np ldb: correction sweep vectors
+};
+
+struct SolveShared {
+    cfl cL[NEGF_MAXNP4 * NEGF_NRH];   // one right-hand side, rows permuted: cL[sigma(i) 16 + b] = v(i, b)
+    cd xL[64 * NEGF_NRH];             // 64 rows of X_m for the residual
+};
+
+// out = g_m v for the 16 columns staged in cL, on the FP32 matrix cores.  Zg is the stored
+// generator (g_m(:, c) = Zg(:, sigma_m(c))), so out(r, b) = sum_c Zg(r, c) cL(c, b).
+// Wave w computes rows [32 w, 32 w + 32) as two 16-row tiles with A = Zg tile
+// (A[i][k] at lane i + 16 k) and B = cL (B[k][b] at lane b + 16 k); lane b + 16 q,
+// register r of tile t holds out(32 w + 16 t + 4 q + r, b).
+__device__ __forceinline__ void gen_matvec(const cfl *Zg, const cfl *cL, int Np, negf_f4 (&ore)[2],
+                                           negf_f4 (&oim)[2]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = lane & 15, k = lane >> 4;
+    const __amdgpu_buffer_rsrc_t zr = negf_rsrc(const_cast<cfl *>(Zg), (unsigned)(Np * Np) * 8u);
+    const unsigned vo = (unsigned)(((w << 5) + i) + k * Np) * 8u;
+    for (int tt = 0; tt < 2; ++tt) {
+        ore[tt] = negf_f4{0.f, 0.f, 0.f, 0.f};
+        oim[tt] = negf_f4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll 4
+    for (int K = 0; K < Np; K += 4) {
+        const unsigned so = (unsigned)K * (unsigned)Np * 8u;
+        const cfl a0 = GjePrec<float>::load(zr, vo, so), a1 = GjePrec<float>::load(zr, vo + 128u, so);
+        const cfl b = cL[(K + k) * NEGF_NRH + i];
+        const float nby = -b.y;
+        ore[0] = GjePrec<float>::mfma(a0.x, b.x, ore[0]);
+        ore[1] = GjePrec<float>::mfma(a1.x, b.x, ore[1]);
+        oim[0] = GjePrec<float>::mfma(a0.x, b.y, oim[0]);
+        oim[1] = GjePrec<float>::mfma(a1.x, b.y, oim[1]);
+        ore[0] = GjePrec<float>::mfma(a0.y, nby, ore[0]);
+        ore[1] = GjePrec<float>::mfma(a1.y, nby, ore[1]);
+        oim[0] = GjePrec<float>::mfma(a0.y, b.x, oim[0]);
+        oim[1] = GjePrec<float>::mfma(a1.y, b.x, oim[1]);
+    }
+}
+
+// Correction right-hand side of slice m (1-based), scattered into cL by sigma_m:
+//     c_m = E_1 delta_{m,1} - X_{m-1} - D_m X_m - X_{m+1} - y_{m+1}     (FP64, rounded)
+// Row i belongs to the lane pair t = 2 i, 2 i + 1; lane h = t & 1 takes the columns
+// j = h, h + 2, ... of D_m (each D entry is built once) for all NB right-hand sides, and
+// the two partial sums meet through a DPP lane swap.
+template <int NB>
+__device__ __forceinline__ void resid_scatter_nb(const SysView &sv, const JobView &jv, const int m, const cd *X,
+                                                 const cfl *Y, const int ldb, const int nR, const int *Rc,
+                                                 const int *sg, SolveShared &so) {
+    const int Np = sv.Np, N = sv.N, Nz = sv.Nz;
+    const int t = threadIdx.x, i = t >> 1, h = t & 1;
+    const int slice = m - 1;
+    const int W = 2 * sv.NGX + 1;
+    const cd *__restrict__ vs = sv.vrec + (size_t)slice * W * (2 * sv.NGY + 1);
+    const bool rin = i < N;
+    const int nxr = rin ? sv.nx[i] : 0, nyr = rin ? sv.ny[i] : 0;
+    const double kin = rin ? jv.kinh[i] : 0.0;
+    cd dg = cmk(0.0, 0.0);
+    if (rin) {
+        if (slice == 0) dg = cadd(dg, jv.sigL[i]);
+        if (slice == Nz - 1) dg = cadd(dg, jv.sigR[i]);
+    }
+    const cd *xm = X + (size_t)slice * Np * ldb;
+    cd acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = cmk(0.0, 0.0);
+    for (int j0 = 0; j0 < N; j0 += 64) {
+        for (int e = t; e < 64 * NEGF_NRH; e += blockDim.x) {
+            const int row = j0 + (e >> 4), b = e & 15;
+            so.xL[e] = (row < N && b < ldb) ? xm[(size_t)row * ldb + b] : cmk(0.0, 0.0);
+        }
+        __syncthreads();
+        if (rin) {
+            const int jn = min(64, N - j0);
+            for (int jj = h; jj < jn; jj += 2) {
+                const int col = j0 + jj;
+                const int ix = (nxr - sv.nx[col] + sv.NGX) + W * (nyr - sv.ny[col] + sv.NGY);
+                const cd d = d_entry(vs[ix], i == col, kin, jv.z, sv.scale, dg);
+                const cd *xr = so.xL + jj * NEGF_NRH;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) acc[b] = cfma(d, xr[b], acc[b]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = cadd(acc[b], lane_swap(acc[b]));
+    if (i < Np) {
+        const int si = sg[i];
+#pragma unroll
+        for (int bb = 0; bb < NEGF_NRH / 2; ++bb) {
+            const int b = 2 * bb + h;                 // lane h writes the columns b % 2 == h
+            cfl v = cmk(0.0f, 0.0f);
+            if (b < nR && rin) {
+                cd ax = cmk(0.0, 0.0);
+#pragma unroll
+                for (int q = 0; q < NB; ++q)
+                    if (q == b) ax = acc[q];
+                cd r = cmk(-ax.x, -ax.y);
+                if (m == 1 && i == Rc[b]) r.x += 1.0;
+                if (m > 1) r = csub(r, X[((size_t)(slice - 1) * Np + i) * ldb + b]);
+                if (m < Nz) {
+                    r = csub(r, X[((size_t)(slice + 1) * Np + i) * ldb + b]);
+                    r = csub(r, to_cd(Y[((size_t)(slice + 1) * Np + i) * ldb + b]));
+                }
+                v = from_cd<cfl>(r);
+            }
+            so.cL[si * NEGF_NRH + b] = v;
+        }
+    }
+}
+
+__device__ __forceinline__ void resid_scatter(const SysView &sv, const JobView &jv, const int m, const cd *X,
+                                              const cfl *Y, const int ldb, const int nR, const int *Rc,
+                                              const int *sg, SolveShared &so) {
+    if (nR <= 2) resid_scatter_nb<2>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
+    else if (nR <= 4) resid_scatter_nb<4>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
+    else if (nR <= 6) resid_scatter_nb<6>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
+    else if (nR <= 8) resid_scatter_nb<8>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
+    else if (nR <= 10) resid_scatter_nb<10>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
+    else if (nR <= 12) resid_scatter_nb<12>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
+    else resid_scatter_nb<16>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
+}
+
+// cL[sigma(i) 16 + b] = coef * V(i, b) (b < nR, else 0) for a row-major Np x ldb array V.
+template <class C>
+__device__ __forceinline__ void scatter_rhs(cfl *cL, const C *V, const int *sg, int Np, int ldb, int nR, float coef) {
+    for (int e = threadIdx.x; e < Np * NEGF_NRH; e += blockDim.x) {
+        const int i = e >> 4, b = e & 15;
+        cfl v = cmk(0.0f, 0.0f);
+        if (b < nR) {
+            const cd x = to_cd(V[(size_t)i * ldb + b]);
+            v = cmk((float)x.x * coef, (float)x.y * coef);
+        }
+        cL[sg[i] * NEGF_NRH + b] = v;
+    }
+}
+
+template <int MAXT>
+__global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
+    __shared__ Gje4Shared<32, float> s;
+    __shared__ SolveShared so;
+    const TransArgs &a = ma.a;
+    const SysView &sv = a.sv;
+    const int Np = sv.Np, Nz = sv.Nz, ldb = ma.ldb;
+    const size_t NN = (size_t)Np * Np;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int jj = lane & 15, q = lane >> 4;
+    cfl *hist = ma.hist + (size_t)blockIdx.x * Nz * NN;
+    int *hsig = ma.hsig + (size_t)blockIdx.x * Nz * Np;
+    cfl *scr = ma.scr + (size_t)blockIdx.x * 2 * 32 * Np;
+    cd *X = ma.X + (size_t)blockIdx.x * Nz * Np * ldb;
+    cfl *Y = ma.Y + (size_t)blockIdx.x * Nz * Np * ldb;
+    for (int job = blockIdx.x; job < a.njobs; job += gridDim.x) {
+        const int nL = a.nL[job], nR = a.nR[job], off = a.open_off[job];
+        if (nL == 0 || nR == 0) {             // no open channel: tau = 0 exactly
+            if (t == 0) {
+                a.tau[job] = 0.0;
+                a.status[job] = 0;
+            }
+            continue;
+        }
+        JobView jv;
+        jv.z = a.z[job];
+        jv.kinh = a.kinh + (size_t)a.kidx[job] * Np;
+        jv.sigL = a.sigL + (size_t)job * Np;
+        jv.sigR = a.sigR + (size_t)job * Np;
+        const int *Rc = a.Rcol + off;
+        int bad = 0;
+        // 1. FP32 sweep, every generator kept: g_m = (D_m - g_{m+1})^{-1}
+        for (int m = Nz; m >= 1; --m) {
+            cfl *M = hist + (size_t)(m - 1) * NN;
+            build_block<cfl>(M, sv, jv, m - 1, m == Nz ? nullptr : hist + (size_t)m * NN, s.sigma);
+            __syncthreads();
+            if (gje_invert4<32, float>(M, Np, s, scr) && !bad) bad = m;
+            for (int i = t; i < Np; i += blockDim.x) hsig[(size_t)(m - 1) * Np + i] = s.sigma[i];
+        }
+        __syncthreads();
+        // 2. first solve: X_1 = g_1(:, open) = Zg_1(:, sigma_1(open)), X_m = -g_m X_{m-1}
+        for (int e = t; e < Np * ldb; e += blockDim.x) {
+            const int i = e / ldb, b = e % ldb;
+            X[e] = b < nR ? to_cd(hist[i + (size_t)hsig[Rc[b]] * Np]) : cmk(0.0, 0.0);
+        }
+        __syncthreads();
+        for (int m = 2; m <= Nz; ++m) {
+            scatter_rhs(so.cL, X + (size_t)(m - 2) * Np * ldb, hsig + (size_t)(m - 1) * Np, Np, ldb, nR, -1.0f);
+            __syncthreads();
+            negf_f4 ore[2], oim[2];
+            gen_matvec(hist + (size_t)(m - 1) * NN, so.cL, Np, ore, oim);
+            if (jj < nR) {
+#pragma unroll
+                for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = (w << 5) + 16 * tt + 4 * q + r;
+                        X[((size_t)(m - 1) * Np + row) * ldb + jj] = cmk((double)ore[tt][r], (double)oim[tt][r]);
+                    }
+            }
+            __syncthreads();
+        }
+        // 3. iterative refinement with FP64 residuals, until the correction of X_{N_z} on
+        // the open rows is below 1e-10 of |X_{N_z}| there (at most ma.nref iterations;
+        // a job that does not get there reports status -1)
+        bool conv = false;
+        for (int it = 0; it < ma.nref && !conv; ++it) {
+            for (int m = Nz; m >= 1; --m) {           // y_m = g_m (r_m - y_{m+1})
+                resid_scatter(sv, jv, m, X, Y, ldb, nR, Rc, hsig + (size_t)(m - 1) * Np, so);
+                __syncthreads();
+                negf_f4 ore[2], oim[2];
+                gen_matvec(hist + (size_t)(m - 1) * NN, so.cL, Np, ore, oim);
+                if (jj < nR) {
+#pragma unroll
+                    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = (w << 5) + 16 * tt + 4 * q + r;
+                            Y[((size_t)(m - 1) * Np + row) * ldb + jj] = cmk(ore[tt][r], oim[tt][r]);
+                        }
+                }
+                __syncthreads();
+            }
+            for (int e = t; e < Np * ldb; e += blockDim.x) X[e] = cadd(X[e], to_cd(Y[e]));   // d_1 = y_1
+            __syncthreads();
+            for (int m = 2; m <= Nz; ++m) {           // d_m = y_m - g_m d_{m-1}, X_m += d_m
+                scatter_rhs(so.cL, Y + (size_t)(m - 2) * Np * ldb, hsig + (size_t)(m - 1) * Np, Np, ldb, nR, 1.0f);
+                __syncthreads();
+                negf_f4 ore[2], oim[2];
+                gen_matvec(hist + (size_t)(m - 1) * NN, so.cL, Np, ore, oim);
+                if (jj < nR) {
+#pragma unroll
+                    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = (w << 5) + 16 * tt + 4 * q + r;
+                            const size_t e = ((size_t)(m - 1) * Np + row) * ldb + jj;
+                            const cfl y = Y[e];
+                            const cfl d = cmk(y.x - ore[tt][r], y.y - oim[tt][r]);
+                            Y[e] = d;
+                            X[e] = cadd(X[e], to_cd(d));
+                        }
+                }
+                __syncthreads();
+            }
+            // convergence: max |d_{N_z}| against max |X_{N_z}| over the open rows
+            double dmx = 0.0, xmx = 0.0;
+            for (int e = t; e < nL * nR; e += blockDim.x) {
+                const int ai = e % nL, bi = e / nL;
+                const size_t o = ((size_t)(Nz - 1) * Np + a.Lrow[off + ai]) * ldb + bi;
+                const cfl d = Y[o];
+                const cd x = X[o];
+                dmx = fmax(dmx, fabs((double)d.x) + fabs((double)d.y));
+                xmx = fmax(xmx, fabs(x.x) + fabs(x.y));
+            }
+            dmx = block_max2(dmx, s.red);
+            xmx = block_max2(xmx, s.red);
+            conv = dmx <= 1e-10 * xmx;
+        }
+        // 4. trace on X_{N_z} = G_{N_z,1}(:, open)
+        double part = 0.0;
+        for (int e = t; e < nL * nR; e += blockDim.x) {
+            const int ai = e % nL, bi = e / nL;
+            const cd v = X[((size_t)(Nz - 1) * Np + a.Lrow[off + ai]) * ldb + bi];
+            part += a.gL[off + ai] * a.gR[off + bi] * (v.x * v.x + v.y * v.y);
+        }
+        const double tot = block_sum2(part, s.red);
+        if (t == 0) {
+            a.tau[job] = tot * sv.scale * sv.scale;
+            a.status[job] = bad ? bad : (conv ? 0 : -1);
         }
         __syncthreads();
     }
@@ -475,6 +770,8 @@ struct negf_ctx {
     // reusable device buffers
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    int mixed_refine = 12;  // refinement iteration limit of NEGF_PREC_MIXED_FP32
+    int last_fallbacks = 0; // energies of the last mixed call redone in FP64
 };
 
 namespace {
@@ -651,6 +948,8 @@ extern "C" {
 const char *negf_version(void) { return "negf_gpu 0.1 (gfx950, FP64 RGF)"; }
 const char *negf_last_error(void) { return g_err.c_str(); }
 
+int negf_last_fallbacks(const negf_ctx *c) { return c ? c->last_fallbacks : 0; }
+
 int negf_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -682,6 +981,7 @@ int negf_ctx_create(negf_ctx **out, int N, int N_z, const int *nx_grid, const in
     c->NGX = NGX;
     c->NGY = NGY;
     c->flags = flags;
+    if (const char *r = std::getenv("NEGF_MIXED_REFINE")) c->mixed_refine = std::max(0, std::atoi(r));
     c->Lx = Lx;
     c->Ly = Ly;
     c->Lz = Lz;
@@ -727,6 +1027,31 @@ int negf_ctx_destroy(negf_ctx *c) {
 }
 
 double negf_last_kernel_seconds(negf_ctx *c) { return c ? c->last_kernel_s : -1.0; }
+
+}  // extern "C"
+
+// Copies tau and the per-job status back, records the launch time and sums
+// tau(E) += w_k tau(E, k) (src/negf.f90:172).
+static int finish_transmission(negf_ctx *c, int nj, const double *d_tau, const int *d_st, const std::vector<int> &je,
+                               const std::vector<int> &jk, const double *kw, double *tau, double *tau_jobs) {
+    std::vector<double> h_tau(nj);
+    std::vector<int> h_st(nj);
+    HIPCHK(hipMemcpyAsync(h_tau.data(), d_tau, nj * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(h_st.data(), d_st, nj * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_kernel_s = ms * 1e-3;
+    int status = 0;
+    for (int j = 0; j < nj; ++j) {
+        tau[je[j]] += h_tau[j] * kw[jk[j]];
+        if (tau_jobs) tau_jobs[j] = h_tau[j];
+        if (h_st[j] && !status) status = h_st[j];
+    }
+    return status;
+}
+
+extern "C" {
 
 int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *kx,
                             const double *ky, const double *kw, int nk, double *tau,
@@ -798,6 +1123,101 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
         return st;
     const int threads = threads_for_shape(c->Np);
     const Shape shp = shape_for(c->Np);
+    if ((c->flags & 3) == NEGF_PREC_MIXED_FP32) {
+        int nRmax = 1;
+        for (int j = 0; j < nj; ++j) nRmax = std::max(nRmax, nR[j]);
+        if (shp == SHAPE_1024_ONE)
+            return fail(-2, "mixed precision needs N <= 512 (padded block " + std::to_string(c->Np) + ")");
+        if (nRmax > NEGF_NRH)
+            return fail(-2, "mixed precision handles at most " + std::to_string(NEGF_NRH) + " open channels, got " +
+                                std::to_string(nRmax));
+        auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t Np = c->Np, Nz = c->Nz, ldb = nRmax;
+        const size_t b_hist = al(Nz * Np * Np * sizeof(cfl)), b_sig = al(Nz * Np * sizeof(int)),
+                     b_scr = al(2 * 32 * Np * sizeof(cfl)), b_x = al(Nz * Np * ldb * sizeof(cd)),
+                     b_y = al(Nz * Np * ldb * sizeof(cfl));
+        const size_t per_slot_m = b_hist + b_sig + b_scr + b_x + b_y;
+        const void *kfm = shp == SHAPE_640 ? (const void *)k_transmission_mixed<640> : (const void *)k_transmission_mixed<1024>;
+        const int slots = slots_for(c, kfm, threads, nj, per_slot_m);
+        if ((st = ensure_ws(c, per_slot_m * slots))) return st;
+        MixedArgs ma;
+        ma.a = a;
+        ma.a.z = d_z;
+        ma.a.kidx = d_k;
+        ma.a.kinh = d_kin;
+        ma.a.sigL = d_sl;
+        ma.a.sigR = d_sr;
+        ma.a.open_off = d_off;
+        ma.a.nL = d_nL;
+        ma.a.nR = d_nR;
+        ma.a.Lrow = d_Lr;
+        ma.a.Rcol = d_Rc;
+        ma.a.gL = d_gl;
+        ma.a.gR = d_gr;
+        ma.a.nLmax = nLmax;
+        ma.a.ws = nullptr;
+        ma.a.tau = d_tau;
+        ma.a.status = d_st;
+        ma.nref = c->mixed_refine;
+        ma.ldb = nRmax;
+        char *base = (char *)c->ws;
+        ma.hist = (cfl *)base;
+        ma.hsig = (int *)(base + b_hist * slots);
+        ma.scr = (cfl *)(base + (b_hist + b_sig) * slots);
+        ma.X = (cd *)(base + (b_hist + b_sig + b_scr) * slots);
+        ma.Y = (cfl *)(base + (b_hist + b_sig + b_scr + b_x) * slots);
+        // each region holds `slots` padded per-slot blocks; the kernel steps through a region
+        // with the unpadded per-slot size, which is never larger
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        if (shp == SHAPE_640) hipLaunchKernelGGL((k_transmission_mixed<640>), dim3(slots), dim3(threads), 0, c->stream, ma);
+        else hipLaunchKernelGGL((k_transmission_mixed<1024>), dim3(slots), dim3(threads), 0, c->stream, ma);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        std::vector<double> h_tau(nj);
+        std::vector<int> h_st(nj);
+        HIPCHK(hipMemcpyAsync(h_tau.data(), d_tau, nj * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(h_st.data(), d_st, nj * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        // energies with a job whose refinement did not converge run again on the FP64 path
+        std::vector<int> redo;
+        {
+            std::vector<char> mark(nE, 0);
+            for (int j = 0; j < nj; ++j)
+                if (h_st[j] < 0) mark[je[j]] = 1;
+            for (int i = 0; i < nE; ++i)
+                if (mark[i]) redo.push_back(i);
+        }
+        double extra_s = 0.0;
+        if (!redo.empty()) {
+            const int n = (int)redo.size();
+            std::vector<double> Er(n), tr(n, 0.0), tj((size_t)n * nk, 0.0);
+            for (int q = 0; q < n; ++q) Er[q] = E[redo[q]];
+            const int saved = c->flags;
+            c->flags &= ~3;
+            const int st2 = negf_transmission_batch(c, Er.data(), n, kx, ky, kw, nk, tr.data(), tj.data());
+            c->flags = saved;
+            if (st2 < 0) return st2;
+            extra_s = c->last_kernel_s;
+            for (int q = 0; q < n; ++q)
+                for (int k = 0; k < nk; ++k) {
+                    const int j = k * nE + redo[q];
+                    h_tau[j] = tj[(size_t)k * n + q];
+                    h_st[j] = 0;
+                }
+            if (st2 > 0) h_st[(size_t)0 * nE + redo[0]] = st2;
+        }
+        c->last_kernel_s = ms * 1e-3 + extra_s;
+        c->last_fallbacks = (int)redo.size();
+        int status = 0;
+        for (int j = 0; j < nj; ++j) {
+            tau[je[j]] += h_tau[j] * kw[jk[j]];
+            if (tau_jobs) tau_jobs[j] = h_tau[j];
+            if (h_st[j] > 0 && !status) status = h_st[j];
+        }
+        return status;
+    }
     const size_t per_slot =
         ((size_t)2 * c->Np * c->Np + (size_t)2 * nLmax * c->Np + (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
     const void *kfn = shp == SHAPE_640 ? (const void *)k_transmission<640, 32, true>
@@ -827,21 +1247,7 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
     else hipLaunchKernelGGL((k_transmission<1024, 16, false>), dim3(slots), dim3(threads), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
-    std::vector<double> h_tau(nj);
-    std::vector<int> h_st(nj);
-    HIPCHK(hipMemcpyAsync(h_tau.data(), d_tau, nj * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(h_st.data(), d_st, nj * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->last_kernel_s = ms * 1e-3;
-    int status = 0;
-    for (int j = 0; j < nj; ++j) {
-        tau[je[j]] += h_tau[j] * kw[jk[j]];   // src/negf.f90:172
-        if (tau_jobs) tau_jobs[j] = h_tau[j];
-        if (h_st[j] && !status) status = h_st[j];
-    }
-    return status;
+    return finish_transmission(c, nj, d_tau, d_st, je, jk, kw, tau, tau_jobs);
 }
 
 static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &energies,

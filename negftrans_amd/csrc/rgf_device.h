@@ -64,6 +64,18 @@ __device__ __forceinline__ cd crecip_fast(cd b) {
 // |re| + |im|, the pivot measure of LAPACK izamax (dcabs1).
 __device__ __forceinline__ double cabs1(cd a) { return fabs(a.x) + fabs(a.y); }
 
+// complex64 (the single-precision sweep of the mixed path) and conversions
+typedef float2 cfl;
+__device__ __forceinline__ cfl cmk(float re, float im) { return make_float2(re, im); }
+__device__ __forceinline__ cd to_cd(cd v) { return v; }
+__device__ __forceinline__ cd to_cd(cfl v) { return cmk((double)v.x, (double)v.y); }
+template <class C>
+__device__ __forceinline__ C from_cd(cd v);
+template <>
+__device__ __forceinline__ cd from_cd<cd>(cd v) { return v; }
+template <>
+__device__ __forceinline__ cfl from_cd<cfl>(cd v) { return cmk((float)v.x, (float)v.y); }
+
 // Shared-memory scratch of one workgroup.  Sizes are bounded by NEGF_MAXNP.
 #define NEGF_MAXNP 1024
 struct RgfShared {
@@ -308,12 +320,32 @@ struct SysView {
     const cd *vrec;          // V_reciprocal_all, (2NGX+1)(2NGY+1)N_z
 };
 
+// Element (r, col), r, col < N, of D_slice = 2 Delta^2 ((E + i eta) I - H_k) - 2 I (+ lead
+// terms on the end slices), in the operation order of E_minus_H_construction
+// (src/grid.f90:291-331) with H_k(i,j) = V(nx_i - nx_j, ny_i - ny_j, k) + delta_ij kin_i
+// (src/grid.f90:248-259).  h = V(...), dg = the lead terms of row r.
+__device__ __forceinline__ cd d_entry(cd h, bool diag, double kin, cd z, double scale, cd dg) {
+    if (diag) h.x += kin;
+    cd d = cmk(-h.x, -h.y);
+    if (diag) d = cadd(d, z);
+    d.x *= scale;
+    d.y *= scale;
+    if (diag) {
+        d.x -= 2.0;
+        d = cadd(d, dg);
+    }
+    return d;
+}
+
 // M = D_slice - S(:, sig) (S == nullptr: M = D_slice); the padding block is the identity.
-// D_k = 2 Delta^2 ((E + i eta) I - H_k) - 2 I (+ lead terms on the end slices), in the
-// operation order of E_minus_H_construction (src/grid.f90:291-331) with
-// H_k(i,j) = V(nx_i - nx_j, ny_i - ny_j, k) + delta_ij kin_i (src/grid.f90:248-259).
-__device__ __forceinline__ void build_block(cd *__restrict__ M, const SysView &sv, const JobView &jv,
-                                            const int slice, const cd *__restrict__ S, const int *sig) {
+// C = cd stores complex128, C = cfl rounds the FP64 result to complex64.
+template <class T>
+struct negf_id {
+    typedef T type;
+};
+template <class C = cd>
+__device__ __forceinline__ void build_block(C *__restrict__ M, const SysView &sv, const JobView &jv, const int slice,
+                                            const typename negf_id<C>::type *__restrict__ S, const int *sig) {
     const int Np = sv.Np, N = sv.N;
     const int W = 2 * sv.NGX + 1;
     const cd *__restrict__ vs = sv.vrec + (size_t)slice * W * (2 * sv.NGY + 1);
@@ -336,29 +368,20 @@ __device__ __forceinline__ void build_block(cd *__restrict__ M, const SysView &s
                 int ix = 0;
                 if (in) ix = (nxr - sv.nx[col] + sv.NGX) + W * (nyr - sv.ny[col] + sv.NGY);
                 hv[q] = in ? vs[ix] : cmk(0.0, 0.0);
-                sv_[q] = S ? S[r + (size_t)(sig ? sig[col] : col) * Np] : cmk(0.0, 0.0);
+                sv_[q] = S ? to_cd(S[r + (size_t)(sig ? sig[col] : col) * Np]) : cmk(0.0, 0.0);
             }
 #pragma unroll
             for (int q = 0; q < CBB; ++q) {
                 const int col = c0 + q;
                 cd v;
                 if (rin && col < N) {
-                    cd h = hv[q];
-                    if (r == col) h.x += kin;
-                    cd d = cmk(-h.x, -h.y);
-                    if (r == col) d = cadd(d, jv.z);
-                    d.x *= sv.scale;
-                    d.y *= sv.scale;
-                    if (r == col) {
-                        d.x -= 2.0;
-                        d = cadd(d, dg);
-                    }
+                    cd d = d_entry(hv[q], r == col, kin, jv.z, sv.scale, dg);
                     if (S) d = csub(d, sv_[q]);
                     v = d;
                 } else {
                     v = cmk(r == col ? 1.0 : 0.0, 0.0);
                 }
-                M[r + (size_t)col * Np] = v;
+                M[r + (size_t)col * Np] = from_cd<C>(v);
             }
         }
     }
@@ -528,6 +551,19 @@ __device__ __forceinline__ double block_sum2(double v, double *red) {
     __syncthreads();
     double t = 0.0;
     for (int i = 0; i < nw; ++i) t += red[i];
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ double block_max2(double v, double *red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t = fmax(t, red[i]);
     __syncthreads();
     return t;
 }
@@ -897,58 +933,65 @@ __device__ __forceinline__ int gje_invert3(cd *__restrict__ Z, const int Np, Gje
 }
 
 // ====================================================================================
-// Gauss-Jordan inversion with the trailing update on the FP64 matrix cores.
+// Gauss-Jordan inversion with the trailing update on the matrix cores, in FP64
+// (v_mfma_f64_16x16x4f64) or FP32 (v_mfma_f32_16x16x4f32).
 //
 // Np % 32 == 0, Np <= NEGF_MAXNP4, blockDim.x == 2 Np.  Wave w owns rows [32 w, 32 w + 32)
 // as two 16-row tiles.  Lane l = j + 16 k (j = l & 15, k = l >> 4) holds, for tile t,
 // the panel values P[t][q] = Z[32 w + 16 t + j][k0 + 4 q + k] -- exactly the B operand
-// of v_mfma_f64_16x16x4f64 (B[k][j] at lane j + 16 k) for the products
+// of the 16x16x4 MFMA (B[k][j] at lane j + 16 k, both precisions) for the products
 //     Z[rows][ct .. ct+15] += P[rows][4q .. 4q+3] * T[4q .. 4q+3][ct .. ct+15]
-// computed as D = A B with A[i][k] = T[4q + k][ct + i] (lane i + 16 k) and D[i][j] in
-// lane j + 16 (i % 4), register i / 4.  T (the old pivot rows) is staged in LDS.
-// During the pivot steps the panel registers rotate once per four steps, so the current
-// column always sits in P[t][0] (no runtime register indexing).
+// computed as D = A B with A[i][k] = T[4q + k][ct + i] (lane i + 16 k).  D[i][j] sits in
+// lane j + 16 (i % 4), register i / 4 for FP64 and in lane j + 16 (i / 4), register i % 4
+// for FP32, so the lane group q = l >> 4 and accumulator register r hold column
+// ct + CQ q + CR r (GjePrec).  T (the old pivot rows) is staged in LDS.  During the pivot
+// steps the panel registers rotate once per four steps, so the current column always
+// sits in P[t][0] (no runtime register indexing).
 // ====================================================================================
 #define NEGF_MAXNP4 512
 #define NEGF_LDT (NEGF_TCH + 1)   // LDS row stride of the staged pivot rows (padded)
 
 typedef double negf_d4 __attribute__((ext_vector_type(4)));
+typedef float negf_f4 __attribute__((ext_vector_type(4)));
 typedef int negf_i4 __attribute__((ext_vector_type(4)));
+typedef int negf_i2 __attribute__((ext_vector_type(2)));
 
-// Raw buffer access to a complex128 array: one 32-bit VGPR offset per lane plus a
-// wave-uniform SGPR offset, instead of a 64-bit address per access.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t negf_rsrc(cd *p, unsigned bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)bytes, 0x00020000);
+// complex64 arithmetic (same conventions as the complex128 helpers)
+__device__ __forceinline__ cfl cmul(cfl a, cfl b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ cfl cfms(cfl a, cfl b, cfl c) {
+    c.x = fmaf(-a.x, b.x, c.x);
+    c.x = fmaf(a.y, b.y, c.x);
+    c.y = fmaf(-a.x, b.y, c.y);
+    c.y = fmaf(-a.y, b.x, c.y);
+    return c;
 }
-__device__ __forceinline__ cd buf_load_cd(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-    const negf_i4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
-    return cmk(__hiloint2double(v.y, v.x), __hiloint2double(v.w, v.z));
-}
-__device__ __forceinline__ void buf_store_cd(cd x, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-    negf_i4 v;
-    v.x = __double2loint(x.x);
-    v.y = __double2hiint(x.x);
-    v.z = __double2loint(x.y);
-    v.w = __double2hiint(x.y);
-    __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
+__device__ __forceinline__ float cabs1(cfl a) { return fabsf(a.x) + fabsf(a.y); }
+// 1/b: exact power-of-two scaling, v_rcp_f32 and one Newton step; Smith's method for
+// zero, infinite or NaN input.
+__device__ __forceinline__ cfl crecip_fast(cfl b) {
+    const float m = fmaxf(fabsf(b.x), fabsf(b.y));
+    if (!(m > 0.0f) || !(m < INFINITY)) {
+        if (fabsf(b.x) >= fabsf(b.y)) {
+            const float r = b.y / b.x, d = b.x + b.y * r;
+            return cmk(1.0f / d, -r / d);
+        }
+        const float r = b.x / b.y, d = b.x * r + b.y;
+        return cmk(r / d, -1.0f / d);
+    }
+    const int e = __builtin_amdgcn_frexp_expf(m);
+    const float bx = __builtin_amdgcn_ldexpf(b.x, -e), by = __builtin_amdgcn_ldexpf(b.y, -e);
+    const float d = fmaf(bx, bx, by * by);
+    float r = __builtin_amdgcn_rcpf(d);
+    r = fmaf(r, fmaf(-d, r, 1.0f), r);
+    return cmk(__builtin_amdgcn_ldexpf(bx * r, -e), __builtin_amdgcn_ldexpf(-by * r, -e));
 }
 
-template <int PB>
-struct Gje4Shared {
-    int perm[NEGF_MAXNP4];
-    int ipiv[NEGF_MAXNP4];
-    int sigma[NEGF_MAXNP4];
-    int disp[2 * PB];
-    int ndisp;
-    int flag;
-    double rv[2][NEGF_MAXW];
-    int ri[2][NEGF_MAXW];
-    cd cand[2][NEGF_MAXW][PB];   // each wave's best candidate row (panel columns)
-    cd rowC[2][PB];              // row c (the pivot position) before the swap
-    cd colJ[2][NEGF_MAXNP4];     // panel column J of every row
-    cd Tl[PB * NEGF_LDT];        // old pivot rows of one column chunk: Tl[p * LDT + (c - c0)]
-    double red[NEGF_MAXW];
-};
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_max_step(float v) {
+    const int i = __float_as_int(v);
+    const float o = __int_as_float(__builtin_amdgcn_update_dpp(i, i, CTRL, ROWMASK, 0xf, false));
+    return o > v ? o : v;
+}
 
 // Max over the 16 lanes of DPP row `row`, returned wave-uniform.
 __device__ __forceinline__ double row16_max(double v, int row) {
@@ -960,34 +1003,112 @@ __device__ __forceinline__ double row16_max(double v, int row) {
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), row << 4);
     return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ float row16_max(float v, int row) {
+    v = dpp_max_step<0xB1, 0xf>(v);
+    v = dpp_max_step<0x4E, 0xf>(v);
+    v = dpp_max_step<0x141, 0xf>(v);
+    v = dpp_max_step<0x140, 0xf>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), row << 4));
+}
+
+// Raw buffer access: one 32-bit VGPR offset per lane plus a wave-uniform SGPR offset,
+// instead of a 64-bit address per access.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t negf_rsrc(void *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)bytes, 0x00020000);
+}
+
+template <class R>
+struct GjePrec;
+template <>
+struct GjePrec<double> {
+    typedef cd C;
+    typedef negf_d4 Acc;
+    static constexpr unsigned ESZ = 16;       // bytes per complex element
+    static constexpr int CQ = 1, CR = 4;      // accumulator column = CQ q + CR r
+    __device__ static __forceinline__ Acc mfma(double a, double b, Acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ C load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        const negf_i4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        return cmk(__hiloint2double(v.y, v.x), __hiloint2double(v.w, v.z));
+    }
+    __device__ static __forceinline__ void store(C x, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        negf_i4 v;
+        v.x = __double2loint(x.x);
+        v.y = __double2hiint(x.x);
+        v.z = __double2loint(x.y);
+        v.w = __double2hiint(x.y);
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
+    }
+};
+template <>
+struct GjePrec<float> {
+    typedef cfl C;
+    typedef negf_f4 Acc;
+    static constexpr unsigned ESZ = 8;
+    static constexpr int CQ = 4, CR = 1;
+    __device__ static __forceinline__ Acc mfma(float a, float b, Acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ C load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        const negf_i2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        return cmk(__int_as_float(v.x), __int_as_float(v.y));
+    }
+    __device__ static __forceinline__ void store(C x, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        negf_i2 v;
+        v.x = __float_as_int(x.x);
+        v.y = __float_as_int(x.y);
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
+    }
+};
+
+template <int PB, class R = double>
+struct Gje4Shared {
+    typedef typename GjePrec<R>::C C;
+    int perm[NEGF_MAXNP4];
+    int ipiv[NEGF_MAXNP4];
+    int sigma[NEGF_MAXNP4];
+    int disp[2 * PB];
+    int ndisp;
+    int flag;
+    R rv[2][NEGF_MAXW];
+    int ri[2][NEGF_MAXW];
+    C cand[2][NEGF_MAXW][PB];   // each wave's best candidate row (panel columns)
+    C rowC[2][PB];              // row c (the pivot position) before the swap
+    C colJ[2][NEGF_MAXNP4];     // panel column J of every row
+    C Tl[PB * NEGF_LDT];        // old pivot rows of one column chunk: Tl[p * LDT + (c - c0)]
+    double red[NEGF_MAXW];
+};
 
 // One pivot step, column J = 4 g + KJ of the panel; P[.][q] holds panel column
 // 4 ((q + g) % (PB / 4)) + k.
-template <int PB, int KJ>
-__device__ __forceinline__ void gje4_step(cd (&P)[2][PB / 4], const int g, const int k0, const int Np,
-                                          const int w, const int j, const int k, Gje4Shared<PB> &s) {
+template <int PB, int KJ, class R>
+__device__ __forceinline__ void gje4_step(typename GjePrec<R>::C (&P)[2][PB / 4], const int g, const int k0,
+                                          const int Np, const int w, const int j, const int k,
+                                          Gje4Shared<PB, R> &s) {
+    typedef typename GjePrec<R>::C C;
     constexpr int KK = PB / 4;
     const int t = threadIdx.x, lane = t & 63, nw = blockDim.x >> 6;
     const int J = 4 * g + KJ, c = k0 + J, par = J & 1;
     const int r0 = (w << 5) + j, r1 = r0 + 16;
     const bool mine = (k == KJ);
-    double v0 = -1.0, v1 = -1.0;
+    R v0 = R(-1), v1 = R(-1);
     if (mine) {
         if (r0 >= c) {
             v0 = cabs1(P[0][0]);
-            if (!(v0 >= 0.0)) v0 = -0.5;
+            if (!(v0 >= R(0))) v0 = R(-0.5);
         }
         if (r1 >= c) {
             v1 = cabs1(P[1][0]);
-            if (!(v1 >= 0.0)) v1 = -0.5;
+            if (!(v1 >= R(0))) v1 = R(-0.5);
         }
         s.colJ[par][r0] = P[0][0];
         s.colJ[par][r1] = P[1][0];
     }
     // izamax order: largest |re| + |im|, lowest row on ties (tile 0 rows precede tile 1)
-    const double vmax = row16_max(v0 > v1 ? v0 : v1, KJ);
+    const R vmax = row16_max(v0 > v1 ? v0 : v1, KJ);
     int idx = 0x7fffffff;
-    if (vmax > -1.0) {
+    if (vmax > R(-1)) {
         const unsigned long long h0 = __ballot(v0 == vmax), h1 = __ballot(v1 == vmax);
         idx = h0 ? (w << 5) + ((__ffsll((long long)h0) - 1) & 15)
                  : (w << 5) + 16 + ((__ffsll((long long)h1) - 1) & 15);
@@ -1018,14 +1139,14 @@ __device__ __forceinline__ void gje4_step(cd (&P)[2][PB / 4], const int g, const
     }
     __syncthreads();
     // winner over the waves (at most 16, all in DPP row 0): first wave holding the max
-    const double wv = lane < nw ? s.rv[par][lane] : -2.0;
+    const R wv = lane < nw ? s.rv[par][lane] : R(-2);
     const int wi = lane < nw ? s.ri[par][lane] : 0x7fffffff;
-    const double bmax = row16_max(wv, 0);
+    const R bmax = row16_max(wv, 0);
     const int wb = __ffsll((long long)__ballot(wv == bmax)) - 1;
     int p = __builtin_amdgcn_readlane(wi, wb);
     if (p < c || p >= Np) p = c;
-    const cd *prow = (p == c) ? s.rowC[par] : s.cand[par][wb];
-    const cd inv = crecip_fast(prow[J]);
+    const C *prow = (p == c) ? s.rowC[par] : s.cand[par][wb];
+    const C inv = crecip_fast(prow[J]);
     if (t == 0) {
         s.ipiv[c] = p;
         if (p != c) {
@@ -1033,11 +1154,11 @@ __device__ __forceinline__ void gje4_step(cd (&P)[2][PB / 4], const int g, const
             s.perm[c] = s.perm[p];
             s.perm[p] = tt;
         }
-        if (prow[J].x == 0.0 && prow[J].y == 0.0 && s.flag == 0) s.flag = c + 1;
+        if (prow[J].x == R(0) && prow[J].y == R(0) && s.flag == 0) s.flag = c + 1;
     }
     // eliminate column J from both rows of this lane: the pivot row position takes the
     // scaled pivot row, the row that supplied the pivot takes old row c
-    cd m0 = s.colJ[par][r0], m1 = s.colJ[par][r1];
+    C m0 = s.colJ[par][r0], m1 = s.colJ[par][r1];
     const bool piv0 = (r0 == c), piv1 = (r1 == c);
     if (r0 == p && p != c) {
         m0 = s.rowC[par][J];
@@ -1051,43 +1172,46 @@ __device__ __forceinline__ void gje4_step(cd (&P)[2][PB / 4], const int g, const
     }
     if (piv0) {
 #pragma unroll
-        for (int q = 0; q < KK; ++q) P[0][q] = cmk(0.0, 0.0);
+        for (int q = 0; q < KK; ++q) P[0][q] = cmk(R(0), R(0));
     }
     if (piv1) {
 #pragma unroll
-        for (int q = 0; q < KK; ++q) P[1][q] = cmk(0.0, 0.0);
+        for (int q = 0; q < KK; ++q) P[1][q] = cmk(R(0), R(0));
     }
-    const cd f0 = piv0 ? cmk(-inv.x, -inv.y) : cmul(m0, inv);
-    const cd f1 = piv1 ? cmk(-inv.x, -inv.y) : cmul(m1, inv);
+    const C f0 = piv0 ? cmk(-inv.x, -inv.y) : cmul(m0, inv);
+    const C f1 = piv1 ? cmk(-inv.x, -inv.y) : cmul(m1, inv);
 #pragma unroll
     for (int q = 0; q < KK; ++q) {
-        const cd pq = prow[4 * ((q + g) % KK) + k];
-        const cd n0 = cfms(f0, pq, P[0][q]), n1 = cfms(f1, pq, P[1][q]);
+        const C pq = prow[4 * ((q + g) % KK) + k];
+        const C n0 = cfms(f0, pq, P[0][q]), n1 = cfms(f1, pq, P[1][q]);
         P[0][q] = (q == 0 && mine) ? cmk(-f0.x, -f0.y) : n0;
         P[1][q] = (q == 0 && mine) ? cmk(-f1.x, -f1.y) : n1;
     }
 }
 
 // In-place inversion of Z (Np x Np) by a workgroup of 2 Np threads; on return
-// inverse[:, c] = Z[:, s.sigma[c]].  scratch: (2 PB) x Np complex per workgroup.  Returns
-// the 1-based column of the first exactly-zero pivot, or 0.
-template <int PB>
-__device__ __forceinline__ int gje_invert4(cd *__restrict__ Z, const int Np, Gje4Shared<PB> &s,
-                                           cd *__restrict__ scratch) {
+// inverse[:, c] = Z[:, s.sigma[c]].  scratch: (2 PB) x Np elements per workgroup.
+// Returns the 1-based column of the first exactly-zero pivot, or 0.
+template <int PB, class R = double>
+__device__ __forceinline__ int gje_invert4(typename GjePrec<R>::C *__restrict__ Z, const int Np,
+                                           Gje4Shared<PB, R> &s, typename GjePrec<R>::C *__restrict__ scratch) {
     static_assert(PB == 32, "two 16-row tiles per wave hold one 32-column panel");
+    typedef GjePrec<R> G;
+    typedef typename G::C C;
+    typedef typename G::Acc Acc;
     constexpr int KK = PB / 4;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int j = lane & 15, k = lane >> 4;
     const int R0 = w << 5;
     const size_t ld = (size_t)Np;
-    cd *Tg = scratch;                      // old pivot rows: Tg[p + c PB]
-    cd *Dg = scratch + (size_t)PB * Np;    // displaced rows: Dg[d + c PB]
+    C *Tg = scratch;                      // old pivot rows: Tg[p + c PB]
+    C *Dg = scratch + (size_t)PB * Np;    // displaced rows: Dg[d + c PB]
     if (t == 0) s.flag = 0;
     for (int k0 = 0; k0 < Np; k0 += PB) {
-        cd P[2][KK];
+        C P[2][KK];
 #pragma unroll
         for (int q = 0; q < KK; ++q) {
-            const cd *zc = Z + (R0 + j) + (size_t)(k0 + 4 * q + k) * ld;
+            const C *zc = Z + (R0 + j) + (size_t)(k0 + 4 * q + k) * ld;
             P[0][q] = zc[0];
             P[1][q] = zc[16];
         }
@@ -1096,11 +1220,11 @@ __device__ __forceinline__ int gje_invert4(cd *__restrict__ Z, const int Np, Gje
         __syncthreads();
 #pragma unroll 1
         for (int g = 0; g < KK; ++g) {
-            gje4_step<PB, 0>(P, g, k0, Np, w, j, k, s);
-            gje4_step<PB, 1>(P, g, k0, Np, w, j, k, s);
-            gje4_step<PB, 2>(P, g, k0, Np, w, j, k, s);
-            gje4_step<PB, 3>(P, g, k0, Np, w, j, k, s);
-            const cd x0 = P[0][0], x1 = P[1][0];
+            gje4_step<PB, 0, R>(P, g, k0, Np, w, j, k, s);
+            gje4_step<PB, 1, R>(P, g, k0, Np, w, j, k, s);
+            gje4_step<PB, 2, R>(P, g, k0, Np, w, j, k, s);
+            gje4_step<PB, 3, R>(P, g, k0, Np, w, j, k, s);
+            const C x0 = P[0][0], x1 = P[1][0];
 #pragma unroll
             for (int q = 0; q < KK - 1; ++q) {
                 P[0][q] = P[0][q + 1];
@@ -1112,7 +1236,7 @@ __device__ __forceinline__ int gje_invert4(cd *__restrict__ Z, const int Np, Gje
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < KK; ++q) {
-            cd *zc = Z + (R0 + j) + (size_t)(k0 + 4 * q + k) * ld;
+            C *zc = Z + (R0 + j) + (size_t)(k0 + 4 * q + k) * ld;
             zc[0] = P[0][q];
             zc[16] = P[1][q];
         }
@@ -1130,14 +1254,14 @@ __device__ __forceinline__ int gje_invert4(cd *__restrict__ Z, const int Np, Gje
             constexpr int GB = 4;
             const int total = rows * Np;
             for (int e0 = t; e0 < total; e0 += GB * blockDim.x) {
-                cd gv[GB];
+                C gv[GB];
 #pragma unroll
                 for (int gg = 0; gg < GB; ++gg) {
                     const int e = e0 + gg * blockDim.x;
                     const int q = e % rows, col = e / rows;
                     const bool ok = e < total && !(col >= k0 && col < k0 + PB);
                     const int src = ok ? (q < PB ? s.perm[k0 + q] : s.perm[s.disp[q - PB]]) : 0;
-                    gv[gg] = ok ? Z[src + col * ld] : cmk(0.0, 0.0);
+                    gv[gg] = ok ? Z[src + col * ld] : cmk(R(0), R(0));
                 }
 #pragma unroll
                 for (int gg = 0; gg < GB; ++gg) {
@@ -1162,20 +1286,20 @@ __device__ __forceinline__ int gje_invert4(cd *__restrict__ Z, const int Np, Gje
         // Column tiles run in increasing order (panel tiles skipped) and the base values
         // of the next tile are loaded while the current one is on the matrix cores.
         const bool zb = (R0 == k0);
-        const __amdgpu_buffer_rsrc_t zr = negf_rsrc(Z, (unsigned)(Np * Np) * 16u);
-        const unsigned cstr = 64u * (unsigned)Np;          // bytes from column c to c + 4
+        const __amdgpu_buffer_rsrc_t zr = negf_rsrc(Z, (unsigned)(Np * Np) * G::ESZ);
+        const unsigned cstr = (unsigned)G::CR * (unsigned)Np * G::ESZ;   // register r -> r + 1
         auto next_tile = [&](int ct) {                      // last tile: reload itself (unused)
             const int n = (ct + 16 == k0) ? k0 + PB : ct + 16;
             return n < Np ? n : ct;
         };
-        auto voff = [&](int ct) { return (unsigned)((R0 + j) + (ct + k) * Np) * 16u; };
-        cd nb0[4], nb1[4];
+        auto voff = [&](int ct) { return (unsigned)((R0 + j) + (ct + G::CQ * k) * Np) * G::ESZ; };
+        C nb0[4], nb1[4];
         auto load_base = [&](int ct) {
             const unsigned vo = voff(ct);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                nb0[q] = buf_load_cd(zr, vo, q * cstr);
-                nb1[q] = buf_load_cd(zr, vo + 256u, q * cstr);
+                nb0[q] = G::load(zr, vo, q * cstr);
+                nb1[q] = G::load(zr, vo + 16u * G::ESZ, q * cstr);
             }
         };
         load_base(k0 == 0 ? PB : 0);
@@ -1190,37 +1314,37 @@ __device__ __forceinline__ int gje_invert4(cd *__restrict__ Z, const int Np, Gje
             __syncthreads();
             for (int ct = c0; ct < c1; ct += 16) {
                 if (ct >= k0 && ct < k0 + PB) continue;
-                negf_d4 ar0, ai0, ar1, ai1;
+                Acc ar0, ai0, ar1, ai1;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    ar0[q] = zb ? 0.0 : nb0[q].x;
-                    ai0[q] = zb ? 0.0 : nb0[q].y;
-                    ar1[q] = zb ? 0.0 : nb1[q].x;
-                    ai1[q] = zb ? 0.0 : nb1[q].y;
+                    ar0[q] = zb ? R(0) : nb0[q].x;
+                    ai0[q] = zb ? R(0) : nb0[q].y;
+                    ar1[q] = zb ? R(0) : nb1[q].x;
+                    ai1[q] = zb ? R(0) : nb1[q].y;
                 }
                 load_base(next_tile(ct));
                 __builtin_amdgcn_sched_barrier(0);          // keep the prefetch ahead of the MFMAs
                 int lx = threadIdx.x & 63;
                 asm volatile("" : "+v"(lx));                 // recompute, do not keep live
-                const cd *tl = s.Tl + (lx >> 4) * NEGF_LDT + (ct - c0 + (lx & 15));
+                const C *tl = s.Tl + (lx >> 4) * NEGF_LDT + (ct - c0 + (lx & 15));
 #pragma unroll
                 for (int q = 0; q < KK; ++q) {
-                    const cd tv = tl[4 * q * NEGF_LDT];
-                    const double ntim = -tv.y;
-                    ar0 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, P[0][q].x, ar0, 0, 0, 0);
-                    ar1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, P[1][q].x, ar1, 0, 0, 0);
-                    ai0 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.y, P[0][q].x, ai0, 0, 0, 0);
-                    ai1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.y, P[1][q].x, ai1, 0, 0, 0);
-                    ar0 = __builtin_amdgcn_mfma_f64_16x16x4f64(ntim, P[0][q].y, ar0, 0, 0, 0);
-                    ar1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ntim, P[1][q].y, ar1, 0, 0, 0);
-                    ai0 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, P[0][q].y, ai0, 0, 0, 0);
-                    ai1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, P[1][q].y, ai1, 0, 0, 0);
+                    const C tv = tl[4 * q * NEGF_LDT];
+                    const R ntim = -tv.y;
+                    ar0 = G::mfma(tv.x, P[0][q].x, ar0);
+                    ar1 = G::mfma(tv.x, P[1][q].x, ar1);
+                    ai0 = G::mfma(tv.y, P[0][q].x, ai0);
+                    ai1 = G::mfma(tv.y, P[1][q].x, ai1);
+                    ar0 = G::mfma(ntim, P[0][q].y, ar0);
+                    ar1 = G::mfma(ntim, P[1][q].y, ar1);
+                    ai0 = G::mfma(tv.x, P[0][q].y, ai0);
+                    ai1 = G::mfma(tv.x, P[1][q].y, ai1);
                 }
                 const unsigned vo = voff(ct);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    buf_store_cd(cmk(ar0[q], ai0[q]), zr, vo, q * cstr);
-                    buf_store_cd(cmk(ar1[q], ai1[q]), zr, vo + 256u, q * cstr);
+                    G::store(cmk(ar0[q], ai0[q]), zr, vo, q * cstr);
+                    G::store(cmk(ar1[q], ai1[q]), zr, vo + 16u * G::ESZ, q * cstr);
                 }
             }
             __syncthreads();

@@ -145,6 +145,11 @@ module negf_gpu_c
             type(c_ptr), value :: ctx
         end function
 
+        integer(c_int) function negf_last_fallbacks(ctx) bind(C, name="negf_last_fallbacks")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
         integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
             import :: c_size_t, c_ptr
             type(c_ptr), value :: s

@@ -271,6 +271,11 @@ class NegfDevice:
     def last_kernel_seconds(self):
         return self._lib.negf_last_kernel_seconds(self._h)
 
+    @property
+    def last_fallbacks(self):
+        """Energies of the last mixed-precision transmission call recomputed in FP64."""
+        return self._lib.negf_last_fallbacks(self._h)
+
     @staticmethod
     def _kp(kpoints):
         if kpoints is None:

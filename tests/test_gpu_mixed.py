@@ -1,0 +1,82 @@
+"""Mixed-precision transmission (NEGF_PREC_MIXED_FP32) through the C-ABI.
+
+The sweep runs in FP32 and keeps every generator in HBM.  G_{N_z,1} on the open
+channels is then refined in FP64 against the exact block-tridiagonal operator (two
+refinement iterations).  Bars, each written in its assertion:
+
+- against the CPU oracle on small systems: |dT| <= 1e-9 (1 + |T|);
+- against the FP64 GPU path at the metric size (N = 317, N_z = 1000): max |dT| <= 1e-6,
+  the accuracy gate of the mixed-vs-FP64 comparison.  The measured value is below 1e-9.
+"""
+import numpy as np
+import pytest
+
+from oracle import negf_oracle as O
+from tests.test_gpu_parity import device_for, oracle_system, small_system
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ng():
+    from negftrans_amd import negf
+    if negf.device_count() < 1:
+        pytest.fail("no GPU visible to libnegf_gpu.so")
+    return negf
+
+
+def mixed_device(ng, s):
+    a = s.atomic
+    at = ng.Atomic(V_L=a.V_L, V_R=a.V_R, ETA=a.ETA, LX=a.LX, LY=a.LY, LZ=a.LZ, ENCUT=a.ENCUT)
+    return ng.NegfDevice(s.nx, s.ny, at, s.Vrec, s.NGX, s.NGY, flags=ng.PREC_MIXED_FP32)
+
+
+@pytest.mark.parametrize("N_z,seed", [(9, 3), (40, 5), (2, 9)])
+def test_mixed_small_vs_oracle(ng, N_z, seed):
+    s, _ = small_system(N_z=N_z, seed=seed)
+    kin = O.kinetic(s.nx, s.ny, s.atomic.LX, s.atomic.LY, 0.0, 0.0)
+    E = np.linspace(0.05, 0.6, 13)
+    assert max(np.sum(2 * e - kin > 0) for e in E) <= 16
+    ref = O.transmission(s, E)
+    dev = mixed_device(ng, s)
+    got = dev.transmission(E)
+    assert dev.last_status == 0
+    assert np.all(np.abs(got - ref) <= 1e-9 * (1.0 + np.abs(ref)))
+
+
+def test_mixed_qpc_c0_vs_fp64(ng):
+    """Config 0 geometry (N = 61, N_z = 200): mixed against the FP64 GPU path."""
+    from negftrans_amd import workloads
+    wl = workloads.qpc_c0()
+    NGX, NGY = wl.ngx()
+    vrec = ng.local_potential_real_to_planewave(wl.V_real_eV / O.HARTREE, NGX, NGY)
+    E = wl.energies_hartree()[::17][:24]
+    f64 = ng.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY).transmission(E)
+    mix = ng.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY, flags=ng.PREC_MIXED_FP32).transmission(E)
+    assert np.max(np.abs(mix - f64)) <= 1e-9
+
+
+def test_mixed_disorder_c1_gate(ng):
+    """Metric configuration (N = 317, N_z = 1000), 32 energies spread over the 0-30 meV
+    grid: max |T_mixed - T_FP64| <= 1e-6."""
+    from negftrans_amd import workloads
+    wl = workloads.disorder_c1()
+    NGX, NGY = wl.ngx()
+    vrec = ng.local_potential_real_to_planewave(wl.V_real_eV / O.HARTREE, NGX, NGY)
+    E = wl.energies_hartree()[5::128]
+    f64 = ng.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY).transmission(E)
+    dev = ng.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY, flags=ng.PREC_MIXED_FP32)
+    mix = dev.transmission(E)
+    assert dev.last_status == 0
+    dT = np.max(np.abs(mix - f64))
+    print(f"max |dT| mixed vs FP64 over {len(E)} energies: {dT:.3e}")
+    assert dT <= 1e-6
+
+
+def test_mixed_rejects_more_than_16_open_channels(ng):
+    s, _ = small_system(N_z=4, seed=3)
+    kin = O.kinetic(s.nx, s.ny, s.atomic.LX, s.atomic.LY, 0.0, 0.0)
+    E = np.array([1.5])
+    assert np.sum(2 * E[0] - kin > 0) > 16
+    with pytest.raises(RuntimeError, match="open channels"):
+        mixed_device(ng, s).transmission(E)
