@@ -405,7 +405,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
             __syncthreads();
         }
         // 3. iterative refinement with FP64 residuals, until the correction of X_{N_z} on
-        // the open rows is below 1e-10 of |X_{N_z}| there (at most ma.nref iterations;
+        // the open rows is below 1e-9 of |X_{N_z}| there (at most ma.nref iterations;
         // a job that does not get there reports status -1)
         bool conv = false;
         for (int it = 0; it < ma.nref && !conv; ++it) {
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
             }
             dmx = block_max2(dmx, s.red);
             xmx = block_max2(xmx, s.red);
-            conv = dmx <= 1e-10 * xmx;
+            conv = dmx <= 1e-9 * xmx;
         }
         // 4. trace on X_{N_z} = G_{N_z,1}(:, open)
         double part = 0.0;
@@ -770,7 +770,7 @@ struct negf_ctx {
     // reusable device buffers
     void *ws = nullptr;
     size_t ws_bytes = 0;
-    int mixed_refine = 12;  // refinement iteration limit of NEGF_PREC_MIXED_FP32
+    int mixed_refine = 8;   // refinement iteration limit of NEGF_PREC_MIXED_FP32
     int last_fallbacks = 0; // energies of the last mixed call redone in FP64
 };
 
@@ -920,6 +920,7 @@ int slots_for(negf_ctx *c, const void *kernel, int threads, int njobs, size_t pe
     long slots = (long)c->n_cu * per_cu;
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && per_slot_bytes > 0) {
+        free_b += c->ws_bytes;   // the context's own workspace is reused (or released first)
         const long fit = (long)((free_b * 0.8) / per_slot_bytes);
         slots = std::min(slots, std::max(1L, fit));
     }
