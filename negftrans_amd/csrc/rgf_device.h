@@ -1,17 +1,16 @@
 // rgf_device.h -- device building blocks of the RGF sweep (gfx950, wave64).
 //
-// Matrices are complex128 (double2), column-major, leading dimension Np (N padded to a
-// multiple of 16; the padding block is the identity and stays inert through every
-// operation).  One workgroup owns one matrix at a time; thread r owns row r (blockDim.x
-// = Np rounded up to 64, at most 1024).
+// Matrices are column-major with leading dimension Np: N padded to a multiple of 32 up to
+// 512 (16 above); the padding block is the identity and stays inert through every
+// operation.  One workgroup owns one matrix at a time: 2 Np threads (two lanes per row)
+// for the matrix-core Gauss-Jordan inversion, Np rounded up to 64 (one thread per row)
+// above Np = 512.  Complex128 throughout, except the FP32 sweep of the mixed path.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include <utility>
 
-#define NEGF_PB 16   // Gauss-Jordan panel width (columns factored per pass)
-#define NEGF_CB 8    // column block of the trailing update
 
 typedef double2 cd;
 
@@ -76,190 +75,7 @@ __device__ __forceinline__ cd from_cd<cd>(cd v) { return v; }
 template <>
 __device__ __forceinline__ cfl from_cd<cfl>(cd v) { return cmk((float)v.x, (float)v.y); }
 
-// Shared-memory scratch of one workgroup.  Sizes are bounded by NEGF_MAXNP.
-#define NEGF_MAXNP 1024
-struct RgfShared {
-    int perm[NEGF_MAXNP];    // row permutation of the current panel (new row r <- old perm[r])
-    int ipiv[NEGF_MAXNP];    // pivot row chosen for each column (LAPACK-style interchanges)
-    int sigma[NEGF_MAXNP];   // column map of the finished inverse: inv[:, c] = S[:, sigma[c]]
-    cd rowA[NEGF_PB];
-    cd rowB[NEGF_PB];
-    cd rowR[NEGF_PB];
-    cd T[NEGF_PB * NEGF_CB];
-    double rv[16];
-    int ri[16];
-    double red[16];
-    int flag;
-};
-
-// Workgroup argmax of (v, idx): larger v wins, ties go to the smaller index (izamax).
-__device__ __forceinline__ int block_argmax(double v, int idx, RgfShared &s) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        double ov = __shfl_xor(v, off, 64);
-        int oi = __shfl_xor(idx, off, 64);
-        if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-    }
-    if (lane == 0) { s.rv[w] = v; s.ri[w] = idx; }
-    __syncthreads();
-    double bv = s.rv[0];
-    int bi = s.ri[0];
-    for (int i = 1; i < nw; ++i) {
-        double ov = s.rv[i];
-        int oi = s.ri[i];
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-    }
-    return bi;
-}
-
-__device__ __forceinline__ double block_sum(double v, RgfShared &s) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    __syncthreads();
-    if (lane == 0) s.red[w] = v;
-    __syncthreads();
-    double t = 0.0;
-    for (int i = 0; i < nw; ++i) t += s.red[i];
-    __syncthreads();
-    return t;
-}
-
-// One Gauss-Jordan step on the register-resident panel: pivot search over the rows not
-// yet pivoted, row interchange, scaling of the pivot row, elimination in every other row.
-// J is a template parameter so every access to P is a static register index.
-template <int J>
-__device__ __forceinline__ void gje_step(cd (&P)[NEGF_PB], const int k0, const int Np,
-                                         const bool own, RgfShared &s) {
-    const int r = threadIdx.x;
-    const int c = k0 + J;
-    // eligible rows: r >= c; a NaN entry ranks below every number but above the
-    // ineligible rows, so the pivot is always a valid row index in [c, Np)
-    double v = -1.0;
-    if (own && r >= c) {
-        v = cabs1(P[J]);
-        if (!(v >= 0.0)) v = -0.5;
-    }
-    int p = block_argmax(v, own && r >= c ? r : 0x7fffffff, s);
-    if (p < c || p >= Np) p = c;
-    if (p != c) {
-        if (r == p) {
-#pragma unroll
-            for (int q = 0; q < NEGF_PB; ++q) s.rowA[q] = P[q];
-        }
-        if (r == c) {
-#pragma unroll
-            for (int q = 0; q < NEGF_PB; ++q) s.rowB[q] = P[q];
-        }
-        __syncthreads();
-        if (r == c) {
-#pragma unroll
-            for (int q = 0; q < NEGF_PB; ++q) P[q] = s.rowA[q];
-        }
-        if (r == p) {
-#pragma unroll
-            for (int q = 0; q < NEGF_PB; ++q) P[q] = s.rowB[q];
-        }
-        if (r == 0) {
-            int t = s.perm[c];
-            s.perm[c] = s.perm[p];
-            s.perm[p] = t;
-        }
-    }
-    if (r == 0) s.ipiv[c] = p;
-    if (r == c) {
-        if (P[J].x == 0.0 && P[J].y == 0.0 && s.flag == 0) s.flag = c + 1;
-        const cd inv = crecip(P[J]);
-#pragma unroll
-        for (int q = 0; q < NEGF_PB; ++q) {
-            P[q] = (q == J) ? inv : cmul(P[q], inv);
-            s.rowR[q] = P[q];
-        }
-    }
-    __syncthreads();
-    if (own && r != c) {
-        const cd f = P[J];
-#pragma unroll
-        for (int q = 0; q < NEGF_PB; ++q) {
-            if (q == J) P[q] = cmul(cmk(-f.x, -f.y), s.rowR[J]);
-            else P[q] = cfms(f, s.rowR[q], P[q]);
-        }
-    }
-    __syncthreads();
-}
-
-template <int... Js>
-__device__ __forceinline__ void gje_panel_steps(cd (&P)[NEGF_PB], const int k0, const int Np,
-                                                const bool own, RgfShared &s,
-                                                std::integer_sequence<int, Js...>) {
-    (gje_step<Js>(P, k0, Np, own, s), ...);
-}
-
-// In-place inversion of the Np x Np matrix Z by blocked Gauss-Jordan elimination with
-// partial (row) pivoting -- the same pivot rule as zgetrf (largest |re|+|im| among the
-// rows not yet pivoted, first index on ties), and the same arithmetic count as
-// zgetrf+zgetri (8 N^3 real flops).  Rows are interchanged physically, panel by panel;
-// the column interchanges that undo them are NOT applied: on return
-//     inverse[:, c] = Z[:, s.sigma[c]]
-// Returns the 1-based column of the first exactly-zero pivot, or 0.
-__device__ __forceinline__ int gje_invert(cd *__restrict__ Z, const int Np, RgfShared &s) {
-    const int r = threadIdx.x;
-    const bool own = r < Np;
-    const size_t ld = (size_t)Np;
-    int zero_piv = 0;
-    for (int k0 = 0; k0 < Np; k0 += NEGF_PB) {
-        cd P[NEGF_PB];
-#pragma unroll
-        for (int j = 0; j < NEGF_PB; ++j) P[j] = own ? Z[r + (k0 + j) * ld] : cmk(0.0, 0.0);
-        for (int i = r; i < Np; i += blockDim.x) s.perm[i] = i;
-        __syncthreads();
-        gje_panel_steps(P, k0, Np, own, s, std::make_integer_sequence<int, NEGF_PB>{});
-        // panel columns are final
-        if (own) {
-#pragma unroll
-            for (int j = 0; j < NEGF_PB; ++j) Z[r + (k0 + j) * ld] = P[j];
-        }
-        // trailing update of every other column: rows permuted by this panel's
-        // interchanges, pivot rows replaced, rank-PB update with the factored panel.
-        const bool pivrow = (r >= k0 && r < k0 + NEGF_PB);
-        const int pr = own ? s.perm[r] : 0;
-        for (int cb = 0; cb < Np; cb += NEGF_CB) {
-            if (cb >= k0 && cb < k0 + NEGF_PB) continue;
-            for (int e = r; e < NEGF_PB * NEGF_CB; e += blockDim.x) {
-                const int jj = e % NEGF_PB, cc = e / NEGF_PB;
-                s.T[e] = Z[s.perm[k0 + jj] + (cb + cc) * ld];
-            }
-            cd a[NEGF_CB];
-#pragma unroll
-            for (int cc = 0; cc < NEGF_CB; ++cc)
-                a[cc] = (own && !pivrow) ? Z[pr + (cb + cc) * ld] : cmk(0.0, 0.0);
-            __syncthreads();
-            if (own) {
-#pragma unroll
-                for (int cc = 0; cc < NEGF_CB; ++cc) {
-                    cd acc = a[cc];
-#pragma unroll
-                    for (int jj = 0; jj < NEGF_PB; ++jj) acc = cfma(P[jj], s.T[jj + cc * NEGF_PB], acc);
-                    Z[r + (cb + cc) * ld] = acc;
-                }
-            }
-            __syncthreads();
-        }
-    }
-    if (r == 0) {
-        for (int c = 0; c < Np; ++c) s.sigma[c] = c;
-        for (int c = Np - 1; c >= 0; --c) {
-            const int p = s.ipiv[c];
-            const int t = s.sigma[c];
-            s.sigma[c] = s.sigma[p];
-            s.sigma[p] = t;
-        }
-        zero_piv = s.flag;
-    }
-    __syncthreads();
-    return zero_piv;
-}
+#define NEGF_MAXNP 1024   // largest padded block (one workgroup per matrix)
 
 // C = A * B(:, sigB) (sigB == nullptr: identity), Np x Np, every wave computes 32x32
 // tiles with a 4x4 register block per lane.  The epilogue functor receives
@@ -398,7 +214,10 @@ __device__ __forceinline__ void copy_cols(cd *__restrict__ dst, const cd *__rest
 // ====================================================================================
 // Gauss-Jordan inversion, streaming version.
 //
-// Same pivot rule and arithmetic as gje_invert; reorganised so that the trailing
+// Pivot rule of zgetrf (largest |re|+|im| among the rows not yet pivoted, first index on
+// ties) and the arithmetic count of zgetrf+zgetri; rows are interchanged physically,
+// panel by panel, and the column interchanges are left to the consumer (on return
+// inverse[:, c] = Z[:, s.sigma[c]]).  One thread per row, for Np > 512.  The trailing
 // update -- the O(N^3) part -- needs no workgroup barrier:
 //  * each of the PB pivot steps of a panel takes ONE barrier: every wave publishes its
 //    best candidate (value, row index, and the row itself) plus the old content of row c,
@@ -689,31 +508,6 @@ __device__ __forceinline__ int gje_invert2(cd *__restrict__ Z, const int Np, Gje
     return s.flag;
 }
 
-// ====================================================================================
-// Gauss-Jordan inversion with two threads per row.
-//
-// Same algorithm and pivot rule as gje_invert2, laid out for latency hiding: thread
-// t owns half h = t & 1 of the panel row r = t >> 1 (PB/2 columns), so a workgroup has
-// 2 Np threads -- twice the waves per SIMD at the same panel width -- and each thread
-// keeps half the registers.  The two halves of a row sit in adjacent lanes; partial
-// sums and the pivot-column value are exchanged with a DPP lane swap.
-// ====================================================================================
-template <int PB>
-struct Gje3Shared {
-    int perm[NEGF_MAXNP];
-    int ipiv[NEGF_MAXNP];
-    int sigma[NEGF_MAXNP];
-    int disp[2 * PB];
-    int ndisp;
-    int flag;
-    double rv[2][NEGF_MAXW];
-    int ri[2][NEGF_MAXW];
-    cd cand[2][NEGF_MAXW][PB];
-    cd rowC[2][PB];
-    cd Tl[PB * NEGF_TCH];      // old pivot rows of one column chunk: Tl[j + (c - c0) PB]
-    double red[NEGF_MAXW];
-};
-
 // Value held by the partner lane (t ^ 1).
 __device__ __forceinline__ double lane_swap(double v) {
     const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0xB1, 0xf, 0xf, false);
@@ -721,216 +515,6 @@ __device__ __forceinline__ double lane_swap(double v) {
     return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ cd lane_swap(cd v) { return cmk(lane_swap(v.x), lane_swap(v.y)); }
-
-template <int PB>
-__device__ __forceinline__ void gje3_step(cd (&P)[PB / 2], const int J, const int k0, const int Np,
-                                          const bool own, const int r, const int h, Gje3Shared<PB> &s) {
-    constexpr int HB = PB / 2;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6, nw = blockDim.x >> 6;
-    const int c = k0 + J, par = J & 1;
-    const int jh = J / HB, jq = J % HB;
-    cd pj = cmk(0.0, 0.0);
-#pragma unroll
-    for (int q = 0; q < HB; ++q)
-        if (q == jq) pj = P[q];
-    const cd pjo = lane_swap(pj);
-    const cd pjr = (h == jh) ? pj : pjo;           // P[r][J], known to both lanes of row r
-    double v = -1.0;
-    if (own && r >= c && h == 0) {
-        v = cabs1(pjr);
-        if (!(v >= 0.0)) v = -0.5;
-    }
-    const double vmax = wave_max(v);
-    const unsigned long long hit = __ballot(v == vmax);
-    const int idx = (vmax >= 0.0 || vmax == -0.5) ? (((w << 6) + (__ffsll((long long)hit) - 1)) >> 1) : 0x7fffffff;
-    if (lane == 0) { s.rv[par][w] = vmax; s.ri[par][w] = idx; }
-    if (r == idx) {
-#pragma unroll
-        for (int q = 0; q < HB; ++q) s.cand[par][w][h * HB + q] = P[q];
-    }
-    if (r == c) {
-#pragma unroll
-        for (int q = 0; q < HB; ++q) s.rowC[par][h * HB + q] = P[q];
-    }
-    __syncthreads();
-    // winner over the waves: lane i reads wave i's candidate; the first lane holding the
-    // maximum is the lowest wave, hence the lowest row among equal values
-    const double wv = lane < nw ? s.rv[par][lane] : -2.0;
-    const int wi = lane < nw ? s.ri[par][lane] : 0x7fffffff;
-    const double bmax = wave_max(wv);
-    const int wb = __ffsll((long long)__ballot(wv == bmax)) - 1;
-    int p = __builtin_amdgcn_readlane(wi, wb);
-    if (p < c || p >= Np) p = c;
-    const cd *prow = (p == c) ? s.rowC[par] : s.cand[par][wb];
-    const cd pv = prow[J];
-    const cd inv = crecip_fast(pv);
-    if (t == 0) {
-        s.ipiv[c] = p;
-        if (p != c) {
-            const int tt = s.perm[c];
-            s.perm[c] = s.perm[p];
-            s.perm[p] = tt;
-        }
-        if (pv.x == 0.0 && pv.y == 0.0 && s.flag == 0) s.flag = c + 1;
-    }
-    if (own) {
-        const bool piv = (r == c);
-        const bool takeC = (r == p && r != c);
-        cd mr = pjr;
-        if (takeC) {
-            mr = s.rowC[par][J];
-#pragma unroll
-            for (int q = 0; q < HB; ++q) P[q] = s.rowC[par][h * HB + q];
-        }
-        if (piv) {
-#pragma unroll
-            for (int q = 0; q < HB; ++q) P[q] = cmk(0.0, 0.0);
-        }
-        const cd f = piv ? cmk(-inv.x, -inv.y) : cmul(mr, inv);
-        const cd *ph = prow + h * HB;
-#pragma unroll
-        for (int q = 0; q < HB; ++q) {
-            const cd nv = cfms(f, ph[q], P[q]);
-            P[q] = (h * HB + q == J) ? cmk(-f.x, -f.y) : nv;
-        }
-    }
-}
-
-// In-place inversion of Z (Np x Np, Np a multiple of PB) by a workgroup of 2 * Np
-// threads (rounded up to whole waves); on return inverse[:, c] = Z[:, s.sigma[c]].
-// scratch: (2 PB) x Np complex per workgroup.  Returns the 1-based column of the first
-// exactly-zero pivot, or 0.
-template <int PB>
-__device__ __forceinline__ int gje_invert3(cd *__restrict__ Z, const int Np, Gje3Shared<PB> &s,
-                                           cd *__restrict__ scratch) {
-    constexpr int HB = PB / 2;
-    constexpr int CB = NEGF_CB2;
-    static_assert(CB % 2 == 0 && NEGF_TCH % PB == 0, "block shape");
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int r = t >> 1, h = t & 1;
-    const bool own = r < Np;
-    const size_t ld = (size_t)Np;
-    cd *Tg = scratch;                      // old pivot rows: Tg[j + c PB]
-    cd *Dg = scratch + (size_t)PB * Np;    // displaced rows: Dg[d + c PB]
-    if (t == 0) s.flag = 0;
-    for (int k0 = 0; k0 < Np; k0 += PB) {
-        cd P[HB];
-#pragma unroll
-        for (int j = 0; j < HB; ++j) P[j] = own ? Z[r + (k0 + h * HB + j) * ld] : cmk(0.0, 0.0);
-        for (int i = t; i < Np; i += blockDim.x) s.perm[i] = i;
-        if (t == 0) s.ndisp = 0;
-        __syncthreads();
-#pragma unroll 1
-        for (int J = 0; J < PB; ++J) gje3_step<PB>(P, J, k0, Np, own, r, h, s);
-        __syncthreads();
-        if (own) {
-#pragma unroll
-            for (int j = 0; j < HB; ++j) Z[r + (k0 + h * HB + j) * ld] = P[j];
-            if (h == 0 && r >= k0 + PB && s.perm[r] != r) s.disp[atomicAdd(&s.ndisp, 1)] = r;
-        }
-        __syncthreads();
-        const int nd = s.ndisp;
-        const int rows = PB + nd;
-        {
-            constexpr int GB = 4;
-            const int total = rows * Np;
-            for (int e0 = t; e0 < total; e0 += GB * blockDim.x) {
-                cd gv[GB];
-#pragma unroll
-                for (int g = 0; g < GB; ++g) {
-                    const int e = e0 + g * blockDim.x;
-                    const int q = e % rows, col = e / rows;
-                    const bool ok = e < total && !(col >= k0 && col < k0 + PB);
-                    const int src = ok ? (q < PB ? s.perm[k0 + q] : s.perm[s.disp[q - PB]]) : 0;
-                    gv[g] = ok ? Z[src + col * ld] : cmk(0.0, 0.0);
-                }
-#pragma unroll
-                for (int g = 0; g < GB; ++g) {
-                    const int e = e0 + g * blockDim.x;
-                    const int q = e % rows, col = e / rows;
-                    if (e < total && !(col >= k0 && col < k0 + PB)) {
-                        if (q < PB) Tg[q + (size_t)col * PB] = gv[g];
-                        else Dg[(q - PB) + (size_t)col * PB] = gv[g];
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        for (int e = t; e < nd * Np; e += blockDim.x) {
-            const int q = e % nd, col = e / nd;
-            if (col >= k0 && col < k0 + PB) continue;
-            Z[s.disp[q] + col * ld] = Dg[q + (size_t)col * PB];
-        }
-        __syncthreads();
-        // streamed rank-PB update, one chunk of NEGF_TCH columns at a time: the chunk's
-        // old pivot rows are staged in LDS by the whole workgroup, then every wave streams
-        // its rows through the chunk's column blocks with a two-block register prefetch.
-        // Lane h accumulates its half of the panel columns, the partner's partial sums
-        // arrive through a lane swap, and lane h stores the block columns cc % 2 == h.
-        const bool pivrow = (r >= k0 && r < k0 + PB);
-        const bool ld_ok = own && !pivrow;
-        for (int c0 = 0; c0 < Np; c0 += NEGF_TCH) {
-            const int c1 = min(c0 + NEGF_TCH, Np);
-            if (c0 >= k0 && c1 <= k0 + PB) continue;       // chunk is all panel
-            for (int e = t; e < (c1 - c0) * PB; e += blockDim.x) {
-                const int col = c0 + e / PB;
-                if (!(col >= k0 && col < k0 + PB)) s.Tl[e] = Tg[e + (size_t)c0 * PB];
-            }
-            __syncthreads();
-            auto nxt = [&](int cb) {
-                cb += CB;
-                if (cb >= k0 && cb < k0 + PB) cb = k0 + PB;
-                return cb;
-            };
-            int cb = (c0 >= k0 && c0 < k0 + PB) ? k0 + PB : c0;
-            int cb1 = nxt(cb), cb2 = nxt(cb1);
-            cd a0[CB / 2], a1[CB / 2];
-#pragma unroll
-            for (int i = 0; i < CB / 2; ++i) {
-                a0[i] = (ld_ok && cb < c1) ? Z[r + (cb + 2 * i + h) * ld] : cmk(0.0, 0.0);
-                a1[i] = (ld_ok && cb1 < c1) ? Z[r + (cb1 + 2 * i + h) * ld] : cmk(0.0, 0.0);
-            }
-            while (cb < c1) {
-                cd a2[CB / 2];
-#pragma unroll
-                for (int i = 0; i < CB / 2; ++i)
-                    a2[i] = (ld_ok && cb2 < c1) ? Z[r + (cb2 + 2 * i + h) * ld] : cmk(0.0, 0.0);
-                // four partial sums per column (j mod 4) are independent FMA chains
-#pragma unroll
-                for (int cc = 0; cc < CB; ++cc) {
-                    const cd *tcol = s.Tl + (cb + cc - c0) * PB + h * HB;
-                    cd p4[4] = {cmk(0.0, 0.0), cmk(0.0, 0.0), cmk(0.0, 0.0), cmk(0.0, 0.0)};
-#pragma unroll
-                    for (int j = 0; j < HB; ++j) p4[j & 3] = cfma(P[j], tcol[j], p4[j & 3]);
-                    const cd sacc = cadd(cadd(p4[0], p4[1]), cadd(p4[2], p4[3]));
-                    // a0[cc >> 1] is this lane's base only when cc % 2 == h, the one case stored
-                    const cd tot = cadd(cadd(a0[cc >> 1], sacc), lane_swap(sacc));
-                    if (own && (cc & 1) == h) Z[r + (cb + cc) * ld] = tot;
-                }
-#pragma unroll
-                for (int i = 0; i < CB / 2; ++i) {
-                    a0[i] = a1[i];
-                    a1[i] = a2[i];
-                }
-                cb = cb1;
-                cb1 = cb2;
-                cb2 = nxt(cb2);
-            }
-            __syncthreads();
-        }
-    }
-    if (t == 0) {
-        for (int c = 0; c < Np; ++c) s.sigma[c] = c;
-        for (int c = Np - 1; c >= 0; --c) {
-            const int p = s.ipiv[c];
-            const int tt = s.sigma[c];
-            s.sigma[c] = s.sigma[p];
-            s.sigma[p] = tt;
-        }
-    }
-    __syncthreads();
-    return s.flag;
-}
 
 // ====================================================================================
 // Gauss-Jordan inversion with the trailing update on the matrix cores, in FP64

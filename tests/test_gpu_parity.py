@@ -117,6 +117,48 @@ def test_transmission_two_slices(ng):
     np.testing.assert_allclose(got, ref, rtol=RTOL, atol=1e-13)
 
 
+# ------------------------------------------------------------------ a5, a6, a10 drop-ins
+def test_gpu_blocks(ng):
+    """Hamiltonian_construction, E_minus_H_construction and Transmission_solver drop-ins
+    (context-free forms and the context forms) against the oracle's literal restatement."""
+    import negftrans_amd._lib as L
+    lib = L.load()
+    s, _ = small_system(N_z=5, seed=11, NK=(2, 2), VL=0.01, VR=-0.02)
+    a = s.atomic
+    N, Nz = s.N, s.N_z
+    nx = np.ascontiguousarray(s.nx, dtype=np.int32)
+    ny = np.ascontiguousarray(s.ny, dtype=np.int32)
+    vrec = np.asfortranarray(s.Vrec, dtype=np.complex128)
+    dev = device_for(ng, s)
+    for ik in range(len(s.kx)):
+        kx, ky = float(s.kx[ik]), float(s.ky[ik])
+        Href = O.hamiltonian(s.nx, s.ny, a.LX, a.LY, kx, ky, s.Vrec, s.NGX, s.NGY)   # (N_z, N, N)
+        H = np.zeros((N, N, Nz), dtype=np.complex128, order="F")
+        assert lib.negf_hamiltonian_direct(N, Nz, L.iptr(nx), L.iptr(ny), a.LX, a.LY, kx, ky, L.dptr(vrec),
+                                           s.NGX, s.NGY, L.dptr(H)) == 0
+        assert rel_err(H, np.moveaxis(Href, 0, 2)) < RTOL
+        Hc = np.zeros_like(H)
+        assert lib.negf_hamiltonian(dev._h, kx, ky, L.dptr(Hc)) == 0
+        assert rel_err(Hc, np.moveaxis(Href, 0, 2)) < RTOL
+        z = 0.37 + 1j * a.ETA
+        Dref = O.e_minus_h(Href.copy(), z, s.nx, s.ny, a.LX, a.LY, a.LZ, kx, ky, a.V_L, a.V_R)
+        D = np.zeros_like(H)
+        assert lib.negf_e_minus_h_from_h(N, Nz, L.dptr(H), z.real, z.imag, L.iptr(nx), L.iptr(ny), a.LX, a.LY,
+                                         a.LZ, kx, ky, a.V_L, a.V_R, L.dptr(D)) == 0
+        assert rel_err(D, np.moveaxis(Dref, 0, 2)) < RTOL
+        Dc = np.zeros_like(H)
+        assert lib.negf_e_minus_h(dev._h, z.real, z.imag, kx, ky, L.dptr(Dc)) == 0
+        assert rel_err(Dc, np.moveaxis(Dref, 0, 2)) < RTOL
+        # Transmission_solver on the rescaled first column of the oracle's F sweep
+        G = O.rgf_first_column_last_slice(Dref) * 2.0 * (a.LZ / Nz) ** 2
+        tref = O.transmission_solver(G, z.real, s.nx, s.ny, a.LX, a.LY, a.LZ, Nz, kx, ky, a.V_L, a.V_R)
+        Gf = np.asfortranarray(G)
+        tau = np.zeros(1)
+        assert lib.negf_transmission_trace(N, Nz, L.dptr(Gf), z.real, L.iptr(nx), L.iptr(ny), a.LX, a.LY,
+                                           a.LZ, kx, ky, a.V_L, a.V_R, L.dptr(tau)) == 0
+        assert tau[0] == pytest.approx(tref, rel=RTOL, abs=1e-14)
+
+
 # ------------------------------------------------------------------ workloads
 def oracle_system(wl, vrec, ngx, ngy):
     a = O.atomic_from_input(**{k: v for k, v in wl.inp.items()
