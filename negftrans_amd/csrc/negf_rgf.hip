@@ -74,7 +74,11 @@ struct RowProdShared {
 __device__ __forceinline__ void row_product(cd *__restrict__ Rn, const cd *__restrict__ R, int nL,
                                             int ldR, const cd *__restrict__ S, const int *sig,
                                             int Np, RowProdShared &rs) {
-    const int c = threadIdx.x;
+    // column c of the result belongs to a lane pair (two threads per column when the
+    // workgroup has 2 Np threads, else one); lane h of the pair sums the k = h mod 2 terms
+    const bool pair = (int)blockDim.x >= 2 * Np;
+    const int c = pair ? (threadIdx.x >> 1) : threadIdx.x, h = pair ? (threadIdx.x & 1) : 0;
+    const int step = pair ? 2 : 1;
     const cd *scol = S + (size_t)(c < Np ? sig[c] : 0) * Np;
     for (int a0 = 0; a0 < nL; a0 += NEGF_RA) {
         cd acc[NEGF_RA];
@@ -88,7 +92,7 @@ __device__ __forceinline__ void row_product(cd *__restrict__ Rn, const cd *__res
             __syncthreads();
             if (c < Np) {
                 const int kend = min(64, Np - k0);
-                for (int kk = 0; kk < kend; ++kk) {
+                for (int kk = h; kk < kend; kk += step) {
                     const cd sv = scol[k0 + kk];
 #pragma unroll
                     for (int a = 0; a < NEGF_RA; ++a) acc[a] = cfma(rs.Rs[a + kk * NEGF_RA], sv, acc[a]);
@@ -96,7 +100,11 @@ __device__ __forceinline__ void row_product(cd *__restrict__ Rn, const cd *__res
             }
             __syncthreads();
         }
-        if (c < Np) {
+        if (pair) {
+#pragma unroll
+            for (int a = 0; a < NEGF_RA; ++a) acc[a] = cadd(acc[a], lane_swap(acc[a]));
+        }
+        if (c < Np && h == 0) {
 #pragma unroll
             for (int a = 0; a < NEGF_RA; ++a)
                 if (a0 + a < nL) Rn[(a0 + a) + (size_t)c * ldR] = acc[a];

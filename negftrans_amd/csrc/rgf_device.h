@@ -166,7 +166,12 @@ __device__ __forceinline__ void build_block(C *__restrict__ M, const SysView &sv
     const int W = 2 * sv.NGX + 1;
     const cd *__restrict__ vs = sv.vrec + (size_t)slice * W * (2 * sv.NGY + 1);
     constexpr int CBB = 8;
-    for (int r = threadIdx.x; r < Np; r += blockDim.x) {
+    // every (row, block of CBB columns) pair is one work item, so all threads of the
+    // workgroup take part (two per row at blockDim = 2 Np); rows stay contiguous per
+    // column for coalesced stores
+    const int nitems = Np * (Np / CBB);
+    for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
+        const int r = it % Np, c0 = (it / Np) * CBB;
         const bool rin = r < N;
         const int nxr = rin ? sv.nx[r] : 0, nyr = rin ? sv.ny[r] : 0;
         const double kin = rin ? jv.kinh[r] : 0.0;
@@ -175,7 +180,7 @@ __device__ __forceinline__ void build_block(C *__restrict__ M, const SysView &sv
             if (slice == 0) dg = cadd(dg, jv.sigL[r]);
             if (slice == sv.Nz - 1) dg = cadd(dg, jv.sigR[r]);
         }
-        for (int c0 = 0; c0 < Np; c0 += CBB) {
+        {
             cd hv[CBB], sv_[CBB];
 #pragma unroll
             for (int q = 0; q < CBB; ++q) {
