@@ -6,7 +6,7 @@ module math_kernel
     use negf_gpu_c
     implicit none
     private
-    public GreensFunction_tri_solver, coefficient_simpson, value_simpson, fermi_func
+    public GreensFunction_tri_solver, coefficient_simpson, value_simpson, fermi_func, gauss_legendre
 contains
     ! Blocks of the inverse of tridiag(I, G_inv_D(:,:,k), I): "D" diagonal, "F" first
     ! column, "L" last column (any combination), as src/math_kernel.f90:44-165; the
@@ -85,6 +85,42 @@ contains
         real*8 :: val
         val = (i - 1) * (b - a) / (N - 1) + a
     end function value_simpson
+
+    ! n-point Gauss-Legendre rule on [-1, 1], ascending nodes (extension, no reference
+    ! counterpart): Newton iteration on P_n from the three-term recurrence, started at
+    ! cos(pi (i - 1/4) / (n + 1/2)); w_i = 2 / ((1 - x_i^2) P_n'(x_i)^2).
+    subroutine gauss_legendre(n, x, w)
+        integer, intent(in) :: n
+        real*8, intent(out) :: x(n), w(n)
+        integer :: i, k, it
+        real*8 :: z, p0, p1, p2, dp, dz
+        do i = 1, n
+            z = cos(pi * (i - 0.25D0) / (n + 0.5D0))
+            do it = 1, 100
+                p0 = 1.D0
+                p1 = z
+                do k = 2, n
+                    p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k
+                    p0 = p1
+                    p1 = p2
+                end do
+                dp = n * (z * p1 - p0) / (z * z - 1.D0)
+                dz = p1 / dp
+                z = z - dz
+                if (abs(dz) < 1.D-15) exit
+            end do
+            p0 = 1.D0
+            p1 = z
+            do k = 2, n
+                p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k
+                p0 = p1
+                p1 = p2
+            end do
+            dp = n * (z * p1 - p0) / (z * z - 1.D0)
+            x(n + 1 - i) = z
+            w(n + 1 - i) = 2.D0 / ((1.D0 - z * z) * dp * dp)
+        end do
+    end subroutine gauss_legendre
 
     ! Fermi-Dirac occupation (src/math_kernel.f90:218-223), atomic units.
     function fermi_func(energy, mu, temperature) result(f_E)

@@ -5,18 +5,21 @@
 module negf
     use, intrinsic :: iso_c_binding
     use global
-    use math_kernel, only: coefficient_simpson, value_simpson, fermi_func
+    use math_kernel, only: coefficient_simpson, value_simpson, fermi_func, gauss_legendre
     use negf_gpu_c
     implicit none
     private
     public Equilibrium_Density, Transmission_Coefficient, Local_Density_Of_State
     public Transmission_batch, Density_batch, LDOS_batch, contour_node, negf_release, negf_precision
+    public negf_contour_rule
 
     type(c_ptr), save :: ctx = c_null_ptr
     complex*16, allocatable, save :: key_V(:, :, :)
     integer, allocatable, save :: key_nx(:), key_ny(:)
     real*8, save :: key_p(6) = 0.D0
     integer(c_int), save :: negf_precision = NEGF_PREC_FP64
+    ! 0: the reference's Simpson rule; 1: Gauss-Legendre nodes on the arc and the line
+    integer, save :: negf_contour_rule = 0
 contains
     ! Create (or reuse) the GPU context for this system.
     subroutine bind_system(V_reciprocal_all, nx_grid, ny_grid, atomic)
@@ -62,13 +65,26 @@ contains
         real*8, intent(in) :: min_V
         type(t_parameters), intent(in) :: atomic
         complex*16, intent(out) :: z, w
-        real*8 :: cl, cr, ll, lr, ec, er, th
+        real*8 :: cl, cr, ll, lr, ec, er, th, gx, gw
         cl = min_V - 1.D0 / hartree
         cr = atomic%MU - atomic%GAP
         ll = atomic%MU - atomic%GAP
         lr = atomic%MU + atomic%GAP
         ec = (cr + cl) / 2
         er = (cr - cl) / 2
+        if (negf_contour_rule == 1) then
+            if (i_integral <= N_circle) then
+                call gl_node(i_integral, N_circle, gx, gw)
+                th = pi / 2 * (gx + 1.D0)
+                z = ec + er * exp(cmplx(0.D0, th, kind=8))
+                w = 2.D0 / pi * (pi / 2 * gw) * cmplx(0.D0, 1.D0, kind=8) * er * exp(cmplx(0.D0, th, kind=8))
+            else
+                call gl_node(i_integral - N_circle, N_line, gx, gw)
+                z = cmplx((lr + ll) / 2 + (lr - ll) / 2 * gx, 0.D0, kind=8)
+                w = -(2.D0 / pi * (lr - ll) / 2 * gw * fermi_func(dble(z), atomic%MU, atomic%TEMPERATURE))
+            end if
+            return
+        end if
         if (i_integral <= N_circle) then
             th = value_simpson(i_integral, N_circle, 0.D0, pi)
             z = ec + er * exp(cmplx(0.D0, th, kind=8))
@@ -79,6 +95,16 @@ contains
             w = -(2.D0 / pi * coefficient_simpson(i_integral - N_circle, N_line, ll, lr) * &
                   fermi_func(dble(z), atomic%MU, atomic%TEMPERATURE))
         end if
+    contains
+        subroutine gl_node(i, n, gx, gw)
+            integer, intent(in) :: i, n
+            real*8, intent(out) :: gx, gw
+            real*8, allocatable :: xs(:), ws(:)
+            allocate(xs(n), ws(n))
+            call gauss_legendre(n, xs, ws)
+            gx = xs(i)
+            gw = ws(i)
+        end subroutine gl_node
     end subroutine contour_node
 
     subroutine Equilibrium_Density(i_job, V_reciprocal_all, nx_grid, ny_grid, N_circle, N_line, min_V, &

@@ -8,7 +8,8 @@
 ! one GPU, so the MPI distribution of the reference has no counterpart here.
 !
 ! Extension keys of the namelist (defaults keep reference behaviour):
-!   PRECISION_MODE = 0 (complex128) | 1 (mixed FP32 bulk products)
+!   PRECISION_MODE = 0 (complex128) | 1 (mixed: FP32 sweep + FP64 refinement, transmission)
+!   CONTOUR_QUADRATURE = 'SIMPSON' (the reference's rule) | 'GAUSS-LEGENDRE' (density contour)
 !   FULL_WINDOW    = .false. (copy only |p|<=NGX/2, |q|<=NGY/2 of V-hat, as the reference)
 !   DUMP_FULL      = .false. (also write *_FULL files with 17 significant digits)
 program negftrans_gpu
@@ -27,10 +28,11 @@ program negftrans_gpu
     integer :: LDOS_GRID(3) = -10, NGX = 0, NGY = 0, NKX = 0, NKY = 0
     integer :: N_circle = 40, N_line_per_eV = 500
     integer :: PRECISION_MODE = 0
+    character(len=32) :: CONTOUR_QUADRATURE = 'SIMPSON'
     logical :: FULL_WINDOW = .false., DUMP_FULL = .false.
     namelist /INPUT/ V_L, V_R, MU, ETA, TEMPERATURE, LX, LY, LZ, NGX, NGY, ENCUT, &
         N_circle, N_line_per_eV, GAP, NKX, NKY, VDS, TRANSMISSION_GRID, if_density, if_transmission, &
-        if_LDOS, LDOS_ENERGY_GRID, LDOS_GRID, PRECISION_MODE, FULL_WINDOW, DUMP_FULL
+        if_LDOS, LDOS_ENERGY_GRID, LDOS_GRID, PRECISION_MODE, FULL_WINDOW, DUMP_FULL, CONTOUR_QUADRATURE
 
     type(t_parameters) :: atomic
     type(t_kpointmesh), allocatable :: kpoint(:)
@@ -152,6 +154,15 @@ program negftrans_gpu
     write(16, *) "----------"
     write(16, *) "Transform Potential to plane wave basis..."
     negf_precision = PRECISION_MODE
+    select case (trim(adjustl(CONTOUR_QUADRATURE)))
+    case ('SIMPSON', 'simpson')
+        negf_contour_rule = 0
+    case ('GAUSS-LEGENDRE', 'gauss-legendre')
+        negf_contour_rule = 1
+    case default
+        write(16, *) "ERROR: CONTOUR_QUADRATURE must be SIMPSON or GAUSS-LEGENDRE"
+        stop 1
+    end select
 
     ! V_reciprocal_all for every slice on the GPU, window as the reference copies it
     call wall(t0)

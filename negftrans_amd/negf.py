@@ -128,26 +128,72 @@ def n_line(atomic: Atomic, N_line_per_eV: int) -> int:
     return int(math.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
 
 
-def density_contour(atomic: Atomic, N_circle: int, N_line: int, min_V: float):
+def gauss_legendre(n: int):
+    """Nodes and weights of the n-point Gauss-Legendre rule on [-1, 1]: Newton iteration
+    on P_n from the three-term recurrence, started at the asymptotic root estimate
+    cos(pi (i - 1/4) / (n + 1/2)); w_i = 2 / ((1 - x_i^2) P_n'(x_i)^2)."""
+    if n < 1:
+        raise ValueError("n must be >= 1")
+    x = np.cos(np.pi * (np.arange(1, n + 1) - 0.25) / (n + 0.5))
+    for _ in range(100):
+        p0, p1 = np.ones_like(x), x.copy()
+        for k in range(2, n + 1):
+            p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+        dp = n * (x * p1 - p0) / (x * x - 1.0)
+        dx = p1 / dp
+        x = x - dx
+        if np.max(np.abs(dx)) < 1e-15:
+            break
+    p0, p1 = np.ones_like(x), x.copy()
+    for k in range(2, n + 1):
+        p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+    dp = n * (x * p1 - p0) / (x * x - 1.0)
+    w = 2.0 / ((1.0 - x * x) * dp * dp)
+    order = np.argsort(x)
+    return x[order], w[order]
+
+
+def density_contour(atomic: Atomic, N_circle: int, N_line: int, min_V: float, quadrature: str = "simpson"):
     """Contour nodes z_n (without eta) and weights w_n of Equilibrium_Density
-    (src/negf.f90:36-65, 86-111); Density_Matrix += w_k w_n G_kk(z_n + i eta)."""
+    (src/negf.f90:36-65, 86-111); Density_Matrix += w_k w_n G_kk(z_n + i eta).
+
+    quadrature = "simpson" is the reference's rule (coefficient_simpson / value_simpson,
+    src/math_kernel.f90:167-204).  "gauss-legendre" (an extension) puts N_circle
+    Gauss-Legendre nodes on the arc theta in [0, pi] and N_line on the real segment, with
+    the same integrands."""
     circle_L = min_V - 1.0 / HARTREE
     circle_R = atomic.MU - atomic.GAP
     line_L, line_R = atomic.MU - atomic.GAP, atomic.MU + atomic.GAP
     E_c, E_R = (circle_R + circle_L) / 2, (circle_R - circle_L) / 2
     z = np.empty(N_circle + N_line, dtype=np.complex128)
     w = np.empty_like(z)
-    for i in range(1, N_circle + N_line + 1):
-        if i <= N_circle:
-            th = value_simpson(i, N_circle, 0.0, PI)
-            e = np.exp(1j * th)
-            z[i - 1] = E_c + E_R * e
-            w[i - 1] = 2.0 / PI * coefficient_simpson(i, N_circle, 0.0, PI) * 1j * E_R * e
-        else:
-            E = value_simpson(i - N_circle, N_line, line_L, line_R)
-            z[i - 1] = E
-            w[i - 1] = -(2.0 / PI * coefficient_simpson(i - N_circle, N_line, line_L, line_R)
-                         * fermi_func(E, atomic.MU, atomic.TEMPERATURE))
+    if quadrature == "simpson":
+        for i in range(1, N_circle + N_line + 1):
+            if i <= N_circle:
+                th = value_simpson(i, N_circle, 0.0, PI)
+                e = np.exp(1j * th)
+                z[i - 1] = E_c + E_R * e
+                w[i - 1] = 2.0 / PI * coefficient_simpson(i, N_circle, 0.0, PI) * 1j * E_R * e
+            else:
+                E = value_simpson(i - N_circle, N_line, line_L, line_R)
+                z[i - 1] = E
+                w[i - 1] = -(2.0 / PI * coefficient_simpson(i - N_circle, N_line, line_L, line_R)
+                             * fermi_func(E, atomic.MU, atomic.TEMPERATURE))
+        return z, w
+    if quadrature != "gauss-legendre":
+        raise ValueError(f"unknown contour quadrature {quadrature!r}")
+    if N_circle > 0:
+        x, g = gauss_legendre(N_circle)
+        th = PI / 2 * (x + 1.0)
+        e = np.exp(1j * th)
+        z[:N_circle] = E_c + E_R * e
+        w[:N_circle] = 2.0 / PI * (PI / 2 * g) * 1j * E_R * e
+    if N_line > 0:
+        x, g = gauss_legendre(N_line)
+        half, mid = (line_R - line_L) / 2, (line_R + line_L) / 2
+        E = mid + half * x
+        z[N_circle:] = E
+        w[N_circle:] = -(2.0 / PI * half * g * np.array([fermi_func(e_, atomic.MU, atomic.TEMPERATURE) for e_ in E]))
     return z, w
 
 

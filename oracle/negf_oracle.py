@@ -388,15 +388,29 @@ def transmission(sys: System, energies, faithful_solver: bool = True):
     return T
 
 
-def density_contour(atomic: Atomic, N_circle: int, N_line: int, min_V: float):
+def density_contour(atomic: Atomic, N_circle: int, N_line: int, min_V: float, quadrature: str = "simpson"):
     """Contour nodes and weights of Equilibrium_Density (src/negf.f90:36-65,86-111).
     Returns list of (energy (complex, without eta), weight (complex)) such that
-    Density_Matrix += w_k * weight * G_kk (G_kk already rescaled by 2 Delta^2)."""
+    Density_Matrix += w_k * weight * G_kk (G_kk already rescaled by 2 Delta^2).
+    quadrature="gauss-legendre" (extension, no reference counterpart) maps
+    numpy.polynomial.legendre.leggauss onto the arc theta in [0, pi] and the real segment."""
     circle_L = min_V - 1.0 / HARTREE
     circle_R = atomic.MU - atomic.GAP
     line_L, line_R = atomic.MU - atomic.GAP, atomic.MU + atomic.GAP
     E_c, E_R = (circle_R + circle_L) / 2, (circle_R - circle_L) / 2
     nodes = []
+    if quadrature == "gauss-legendre":
+        xc, wc = np.polynomial.legendre.leggauss(N_circle) if N_circle else ([], [])
+        for x, g in zip(xc, wc):
+            theta = PI / 2 * (x + 1.0)
+            nodes.append((complex(E_c + E_R * np.exp(1j * theta)),
+                          complex(2.0 / PI * (PI / 2 * g) * 1j * E_R * np.exp(1j * theta))))
+        xl, wl = np.polynomial.legendre.leggauss(N_line) if N_line else ([], [])
+        half, mid = (line_R - line_L) / 2, (line_R + line_L) / 2
+        for x, g in zip(xl, wl):
+            E = mid + half * x
+            nodes.append((complex(E), complex(-(2.0 / PI * half * g * fermi_func(E, atomic.MU, atomic.TEMPERATURE)))))
+        return nodes
     for i in range(1, N_circle + N_line + 1):
         if i <= N_circle:
             theta = value_simpson(i, N_circle, 0.0, PI)
@@ -423,12 +437,12 @@ def diagonal_blocks(sys: System, energy: complex, ik: int):
     return tri_solver(D, "D")["diagonal"] * 2.0 * (a.LZ / sys.N_z) ** 2
 
 
-def density_matrix(sys: System, N_circle: int, N_line: int, min_V: float):
+def density_matrix(sys: System, N_circle: int, N_line: int, min_V: float, quadrature: str = "simpson"):
     """Equilibrium_Density summed over all jobs (src/negf.f90:9-113,
     src/main.f90:272-288): Density_Matrix(N, N, N_z) stored as (N_z, N, N)."""
     DM = np.zeros((sys.N_z, sys.N, sys.N), dtype=np.complex128)
     for ik in range(len(sys.kx)):
-        for E, w in density_contour(sys.atomic, N_circle, N_line, min_V):
+        for E, w in density_contour(sys.atomic, N_circle, N_line, min_V, quadrature):
             DM += sys.kw[ik] * w * diagonal_blocks(sys, E, ik)
     return DM
 

@@ -120,3 +120,26 @@ def test_fortran_host_end_to_end(tmp_path):
             allv = np.array([float(x) for x in lines[1:] if x.strip()])
             L = allv.reshape(dims, order="F")
         assert np.linalg.norm(L[:, :, :, ie] - ref) <= 1e-10 * max(np.linalg.norm(ref), 1e-300)
+
+
+@pytest.mark.gpu
+def test_fortran_host_gauss_legendre_density(tmp_path):
+    """CONTOUR_QUADRATURE = 'GAUSS-LEGENDRE' (extension key): DENSITY against the oracle
+    on the same Gauss-Legendre contour (numpy's nodes), relative 1e-10."""
+    N_x, N_y, N_z = 24, 28, 4
+    V_eV = 0.3 * np.random.default_rng(7).standard_normal((N_x, N_y, N_z)) + 1.0
+    inp = dict(CASE, if_density=True, DUMP_FULL=True, CONTOUR_QUADRATURE="'GAUSS-LEGENDRE'")
+    write_case(tmp_path, inp, V_eV)
+    r = run(tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr + open(tmp_path / "OUTPUT").read()
+    s, V = oracle_case(V_eV)
+    a = s.atomic
+    N_line = O.n_line(a, CASE["N_line_per_eV"])
+    DM = O.density_matrix(s, CASE["N_circle"], N_line, float(V.min()), "gauss-legendre")
+    Dref = O.density_real(DM, s.nx, s.ny, N_x, N_y, CASE["LX"], CASE["LY"], CASE["LZ"])
+    Dgot = np.loadtxt(tmp_path / "DENSITY_FULL", skiprows=1).ravel().reshape((N_x, N_y, N_z), order="F")
+    assert np.linalg.norm(Dgot - Dref) <= 1e-10 * np.linalg.norm(Dref)
+    # and it differs from the Simpson density by more than the bar (the key is honoured)
+    DS = O.density_real(O.density_matrix(s, CASE["N_circle"], N_line, float(V.min())), s.nx, s.ny, N_x, N_y,
+                        CASE["LX"], CASE["LY"], CASE["LZ"])
+    assert np.linalg.norm(DS - Dref) > 1e-8 * np.linalg.norm(Dref)

@@ -277,6 +277,46 @@ def test_contour_circle_closure():
     np.testing.assert_allclose(circ, -2 / np.pi * tot, rtol=2e-4, atol=2e-5 * np.abs(tot).max())
 
 
+def test_gauss_legendre_nodes_match_numpy():
+    """The host's Gauss-Legendre rule (own Newton iteration) against
+    numpy.polynomial.legendre.leggauss, and exactness up to degree 2n - 1."""
+    from negftrans_amd import negf
+    for n in (1, 2, 3, 7, 16, 40, 64):
+        x, w = negf.gauss_legendre(n)
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        assert np.max(np.abs(x - xr)) < 1e-14 and np.max(np.abs(w - wr)) < 1e-13
+        for deg in (0, 2 * n - 2, 2 * n - 1):
+            exact = 0.0 if deg % 2 else 2.0 / (deg + 1)
+            assert np.sum(w * x ** deg) == pytest.approx(exact, abs=1e-13)
+
+
+def test_gauss_legendre_contour_converges_to_simpson_reference():
+    """Gauss-Legendre contour (extension): the host and the oracle give the same nodes,
+    and the circle part converges to a dense (N = 301) Simpson reference faster than
+    the reference's Simpson rule with a comparable node count."""
+    from negftrans_amd import negf
+    s, V = tiny_system(N_z=3, seed=5, eta=2e-3)
+    a = s.atomic
+    a.MU, a.TEMPERATURE = 0.6, 300.0
+    a.GAP = 6 * O.K_B * a.TEMPERATURE
+    min_V = -0.2
+    ha = negf.Atomic(MU=a.MU, TEMPERATURE=a.TEMPERATURE, GAP=a.GAP, ETA=a.ETA, LX=a.LX, LY=a.LY, LZ=a.LZ,
+                     ENCUT=a.ENCUT)
+    z, w = negf.density_contour(ha, 12, 9, min_V, "gauss-legendre")
+    nodes = O.density_contour(a, 12, 9, min_V, "gauss-legendre")
+    np.testing.assert_allclose(z, [e for e, _ in nodes], rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(w, [q for _, q in nodes], rtol=1e-13, atol=1e-15)
+
+    def circle(n, quad):
+        nd = O.density_contour(a, n, 3, min_V, quad)[:n]
+        return sum(q * O.diagonal_blocks(s, e, 0) for e, q in nd)
+    ref = circle(301, "simpson")
+    scale = np.abs(ref).max()
+    e_gl = [np.abs(circle(n, "gauss-legendre") - ref).max() / scale for n in (8, 16, 24)]
+    e_si = np.abs(circle(17, "simpson") - ref).max() / scale
+    assert e_gl[0] > e_gl[1] > e_gl[2] and e_gl[2] < 1e-6 and e_gl[1] < e_si / 100
+
+
 def test_ldos_matrix_vs_dense():
     s, V = tiny_system(N_z=4, seed=8, eta=1e-3)
     E = 0.9
