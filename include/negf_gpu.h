@@ -154,6 +154,19 @@ int negf_transmission_trace(int N, int N_z, const double *G, double E, const int
                             const int *ny_grid, double Lx, double Ly, double Lz, double kx,
                             double ky, double V_L, double V_R, double *tau);
 
+/*
+ * Extension, no reference counterpart (the reference rejects VDS != 0,
+ * src/main.f90:129-132): the bias-window part of the non-equilibrium density,
+ *   density_matrix(:, :, k) += sum_{n, kp} kw(kp) w(n) [G Gamma_L G^dagger]_kk (E(n) + i eta)
+ * with G rescaled by 2 Delta^2 (as Equilibrium_Density rescales, src/negf.f90:73-84) and
+ * Gamma_L = diag(sin(Q^L_i Delta) / Delta^2) on slice 1 (left lead, real energy, closed
+ * channels 0, the trace's formula at src/grid.f90:355-360).  E : nE real energies
+ * (Hartree, eta added inside); w : nE complex weights (re, im interleaved);
+ * density_matrix : N*N*N_z complex128, accumulated as negf_density_batch does.
+ */
+int negf_neq_density_batch(negf_ctx *ctx, const double *E, const double *w, int nE, const double *kx,
+                           const double *ky, const double *kw, int nk, double *density_matrix);
+
 /* Wall time (seconds) of the last batched call's device work, from HIP events. */
 double negf_last_kernel_seconds(negf_ctx *ctx);
 

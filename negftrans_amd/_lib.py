@@ -31,6 +31,7 @@ SIGNATURES = {
     "negf_rgf_tri_solve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp]),
     "negf_density_batch": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp]),
     "negf_ldos_batch": (ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp]),
+    "negf_neq_density_batch": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp]),
     "negf_pw_to_real": (ctypes.c_int, [ctypes.c_int, _ip, _ip, ctypes.c_int, _dp, ctypes.c_int,
                                        ctypes.c_int, _dp]),
     "negf_vhat_from_real": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int,

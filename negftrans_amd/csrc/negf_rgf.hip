@@ -564,6 +564,103 @@ __global__ __launch_bounds__(MAXT) void k_diagonal(DiagArgs a) {
     }
 }
 
+// Bias-window spectral blocks for the non-equilibrium density (extension, no reference
+// counterpart; the reference rejects VDS != 0, src/main.f90:129-132).  Per job the
+// backward generators are stored as in D mode, then the first block column on the open
+// channels of the left lead, X_1 = G_11(:, open), X_k = -g_k X_{k-1} (the F recursion
+// of src/math_kernel.f90:120-128), gives
+//     out_k += wgt * sum_a gamma^L_a X_k(:, a) X_k(:, a)^dagger      (r, c < N),
+// i.e. the diagonal blocks of G Gamma_L G^dagger with Gamma_L on slice 1.
+struct NeqArgs {
+    SysView sv;
+    int njobs;
+    const cd *z;
+    const int *kidx;
+    const double *kinh;
+    const cd *sigL, *sigR;
+    const cd *wgt;          // per job: window weight * k weight * (2 Delta^2)^2
+    const int *open_off, *nL, *Lcol;   // open channels of the left lead (flat lists)
+    const double *gL;
+    int nLmax;
+    cd *ws;                 // per slot (N_z + 1) Np^2 + 2 nLmax Np + 2 PB Np
+    cd *out;                // N*N*N_z, column-major ld N
+    int *status;
+};
+
+template <int MAXT, int PB, bool TWO>
+__global__ __launch_bounds__(MAXT) void k_neq(NeqArgs a) {
+    __shared__ GjeSh<PB, TWO> s;
+    const SysView &sv = a.sv;
+    const int Np = sv.Np, N = sv.N, Nz = sv.Nz, ldV = a.nLmax;
+    const size_t NN = (size_t)Np * Np;
+    cd *G = a.ws + (size_t)blockIdx.x * ((Nz + 1) * NN + 2 * (size_t)ldV * Np + 2 * (size_t)PB * Np);
+    cd *W = G + (size_t)Nz * NN;                  // G[0] = G_11, G[m - 1] = g_m (m >= 2)
+    cd *V0 = W + NN, *V1 = V0 + (size_t)ldV * Np;
+    cd *scratch = V1 + (size_t)ldV * Np;
+    for (int job = blockIdx.x; job < a.njobs; job += gridDim.x) {
+        const int nL = a.nL[job], off = a.open_off[job];
+        if (nL == 0) {                            // no open channel of the left lead
+            if (threadIdx.x == 0) a.status[job] = 0;
+            continue;
+        }
+        JobView jv;
+        jv.z = a.z[job];
+        jv.kinh = a.kinh + (size_t)a.kidx[job] * Np;
+        jv.sigL = a.sigL + (size_t)job * Np;
+        jv.sigR = a.sigR + (size_t)job * Np;
+        const cd wgt = a.wgt[job];
+        int bad = 0;
+        for (int m = Nz; m >= 1; --m) {
+            build_block(W, sv, jv, m - 1, m == Nz ? nullptr : G + (size_t)m * NN, nullptr);
+            __syncthreads();
+            if (gje_any<PB, TWO>(W, Np, s, scratch) && !bad) bad = m;
+            copy_cols(G + (size_t)(m - 1) * NN, W, s.sigma, Np);
+            __syncthreads();
+        }
+        auto accumulate = [&](int k, const cd *V) {
+            cd *ok = a.out + (size_t)k * N * N;
+            for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+                const int r = e % N, c = e / N;
+                cd v = cmk(0.0, 0.0);
+                for (int q = 0; q < nL; ++q) {
+                    const cd x = V[(size_t)q * Np + r], y = V[(size_t)q * Np + c];
+                    const double gq = a.gL[off + q];
+                    v.x += gq * (x.x * y.x + x.y * y.y);       // x conj(y)
+                    v.y += gq * (x.y * y.x - x.x * y.y);
+                }
+                const cd t = cmul(wgt, v);
+                double *o = reinterpret_cast<double *>(ok + e);
+                atomicAdd(o, t.x);
+                atomicAdd(o + 1, t.y);
+            }
+        };
+        for (int e = threadIdx.x; e < nL * Np; e += blockDim.x) {
+            const int q = e / Np, r = e % Np;
+            V0[e] = G[r + (size_t)a.Lcol[off + q] * Np];
+        }
+        __syncthreads();
+        accumulate(0, V0);
+        for (int k = 2; k <= Nz; ++k) {
+            const cd *gk = G + (size_t)(k - 1) * NN;
+            for (int e = threadIdx.x; e < nL * Np; e += blockDim.x) {
+                const int q = e / Np, r = e % Np;
+                const cd *vq = V0 + (size_t)q * Np;
+                cd acc = cmk(0.0, 0.0);
+                for (int c = 0; c < Np; ++c) acc = cfma(gk[r + (size_t)c * Np], vq[c], acc);
+                V1[e] = cmk(-acc.x, -acc.y);
+            }
+            __syncthreads();
+            accumulate(k - 1, V1);
+            __syncthreads();
+            cd *t = V0;
+            V0 = V1;
+            V1 = t;
+        }
+        if (threadIdx.x == 0) a.status[job] = bad;
+        __syncthreads();
+    }
+}
+
 // Batched primitives (drop-ins for inverse / matrix_product, src/math_kernel.f90:8-42,
 // and the building blocks of the generic tri-diagonal solver).
 // inv[b] = (A[b] - (Sub ? Sub[b] : 0))^{-1} for count matrices of N x N (ld N on both
@@ -1489,6 +1586,113 @@ int negf_density_batch(negf_ctx *c, const double *z, const double *w, int nnode,
             wg.push_back(std::complex<double>(w[2 * n], w[2 * n + 1]) * kw[ik] * s);
         }
     return diagonal_jobs(c, en, je, jk, wg, tg, 1, kx, ky, nk, density_matrix, true);
+}
+
+int negf_neq_density_batch(negf_ctx *c, const double *E, const double *w, int nE, const double *kx,
+                           const double *ky, const double *kw, int nk, double *density_matrix) {
+    int st = check_ctx(c);
+    if (st) return st;
+    if (nE < 0 || nk < 1 || !kx || !ky || !kw || (nE > 0 && (!E || !w || !density_matrix)))
+        return fail(-2, "bad arguments to negf_neq_density_batch");
+    if (nE == 0) return 0;
+    HIPCHK(hipSetDevice(c->device));
+    const double delta = c->Lz / c->Nz, sc = 2 * delta * delta;
+    const int nj = nE * nk;
+    std::vector<std::complex<double>> energies(nE);
+    std::vector<double> realE(E, E + nE);
+    std::vector<int> je(nj), jk(nj);
+    std::vector<cd> hw(nj);
+    for (int i = 0; i < nE; ++i) energies[i] = E[i];
+    for (int j = 0; j < nj; ++j) {
+        je[j] = j % nE;
+        jk[j] = j / nE;
+        const std::complex<double> wj = std::complex<double>(w[2 * je[j]], w[2 * je[j] + 1]) * kw[jk[j]] * sc * sc;
+        hw[j] = make_double2(wj.real(), wj.imag());
+    }
+    JobTables t;
+    job_tables(c, energies, realE, kx, ky, nk, je, jk, true, t);
+    std::vector<int> off(nj), nL(nj), Lcol;
+    std::vector<double> gLv;
+    int nLmax = 1;
+    for (int j = 0; j < nj; ++j) {
+        off[j] = (int)Lcol.size();
+        int l = 0;
+        for (int i = 0; i < c->N; ++i) {
+            const double g = t.gamL[(size_t)j * c->Np + i];
+            if (g != 0.0) {
+                Lcol.push_back(i);
+                gLv.push_back(g);
+                ++l;
+            }
+        }
+        nL[j] = l;
+        nLmax = std::max(nLmax, l);
+    }
+    if (Lcol.empty()) {
+        Lcol.push_back(0);
+        gLv.push_back(0.0);
+    }
+    Temps tmp;
+    NeqArgs a;
+    a.sv = sysview(c);
+    a.njobs = nj;
+    cd *d_z, *d_sl, *d_sr, *d_w, *d_out;
+    int *d_k, *d_st, *d_off, *d_nL, *d_Lc;
+    double *d_kin, *d_gl;
+    const size_t njp = (size_t)nj * c->Np;
+    const size_t nout = (size_t)c->N * c->N * c->Nz;
+    if ((st = tmp.upload(&d_z, t.z.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_k, t.kidx.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_kin, t.kinh.data(), (size_t)nk * c->Np, c->stream)) ||
+        (st = tmp.upload(&d_sl, t.sigL.data(), njp, c->stream)) ||
+        (st = tmp.upload(&d_sr, t.sigR.data(), njp, c->stream)) ||
+        (st = tmp.upload(&d_w, hw.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_off, off.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_nL, nL.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_Lc, Lcol.data(), Lcol.size(), c->stream)) ||
+        (st = tmp.upload(&d_gl, gLv.data(), gLv.size(), c->stream)) ||
+        (st = tmp.upload(&d_out, reinterpret_cast<const cd *>(density_matrix), nout, c->stream)) ||
+        (st = tmp.alloc(&d_st, nj)))
+        return st;
+    const int threads = threads_for_shape(c->Np);
+    const Shape shp = shape_for(c->Np);
+    const size_t per_slot = ((size_t)(c->Nz + 1) * c->Np * c->Np + (size_t)2 * nLmax * c->Np +
+                             (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
+    const void *kfn = shp == SHAPE_640 ? (const void *)k_neq<640, 32, true>
+                    : shp == SHAPE_1024_TWO ? (const void *)k_neq<1024, 32, true>
+                                            : (const void *)k_neq<1024, 16, false>;
+    const int slots = slots_for(c, kfn, threads, nj, per_slot);
+    if ((st = ensure_ws(c, per_slot * slots))) return st;
+    a.z = d_z;
+    a.kidx = d_k;
+    a.kinh = d_kin;
+    a.sigL = d_sl;
+    a.sigR = d_sr;
+    a.wgt = d_w;
+    a.open_off = d_off;
+    a.nL = d_nL;
+    a.Lcol = d_Lc;
+    a.gL = d_gl;
+    a.nLmax = nLmax;
+    a.ws = (cd *)c->ws;
+    a.out = d_out;
+    a.status = d_st;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if (shp == SHAPE_640) hipLaunchKernelGGL((k_neq<640, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
+    else if (shp == SHAPE_1024_TWO) hipLaunchKernelGGL((k_neq<1024, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_neq<1024, 16, false>), dim3(slots), dim3(threads), 0, c->stream, a);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    std::vector<int> h_st(nj);
+    HIPCHK(hipMemcpyAsync(density_matrix, d_out, nout * sizeof(cd), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(h_st.data(), d_st, nj * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_kernel_s = ms * 1e-3;
+    for (int j = 0; j < nj; ++j)
+        if (h_st[j]) return h_st[j];
+    return 0;
 }
 
 int negf_ldos_batch(negf_ctx *c, const double *E, int nE, const double *kx, const double *ky,

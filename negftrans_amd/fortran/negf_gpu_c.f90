@@ -145,6 +145,15 @@ module negf_gpu_c
             type(c_ptr), value :: ctx
         end function
 
+        integer(c_int) function negf_neq_density_batch(ctx, E, w, nE, kx, ky, kw, nk, dm) &
+                bind(C, name="negf_neq_density_batch")
+            import :: c_int, c_double, c_ptr, c_double_complex
+            type(c_ptr), value :: ctx
+            real(c_double) :: E(*), kx(*), ky(*), kw(*)
+            complex(c_double_complex) :: w(*), dm(*)
+            integer(c_int), value :: nE, nk
+        end function
+
         integer(c_int) function negf_last_fallbacks(ctx) bind(C, name="negf_last_fallbacks")
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx

@@ -197,6 +197,29 @@ def density_contour(atomic: Atomic, N_circle: int, N_line: int, min_V: float, qu
     return z, w
 
 
+def bias_window(atomic: Atomic, VDS: float, n_nodes: int, quadrature: str = "simpson"):
+    """Nodes E_n and weights w_n of the bias-window integral of the non-equilibrium density
+    (extension; the reference rejects VDS != 0, src/main.f90:129-132):
+        Density_Matrix += i (1/pi) integral (f_L - f_R) [G Gamma_L G^dagger] dE
+    with mu_L = MU, mu_R = MU - VDS (VDS > 0) over [mu_R - GAP, mu_L + GAP].  The i puts
+    the term into Im(Density_Matrix), which the density reads."""
+    if VDS <= 0.0:
+        raise ValueError("bias_window needs VDS > 0 (mu_L = MU above mu_R = MU - VDS)")
+    muL, muR = atomic.MU, atomic.MU - VDS
+    lo, hi = muR - atomic.GAP, muL + atomic.GAP
+    if quadrature == "gauss-legendre":
+        x, g = gauss_legendre(n_nodes)
+        E = (hi + lo) / 2 + (hi - lo) / 2 * x
+        q = (hi - lo) / 2 * g
+    elif quadrature == "simpson":
+        E = np.array([value_simpson(i, n_nodes, lo, hi) for i in range(1, n_nodes + 1)])
+        q = np.array([coefficient_simpson(i, n_nodes, lo, hi) for i in range(1, n_nodes + 1)])
+    else:
+        raise ValueError(f"unknown quadrature {quadrature!r}")
+    df = np.array([fermi_func(e, muL, atomic.TEMPERATURE) - fermi_func(e, muR, atomic.TEMPERATURE) for e in E])
+    return E, 1j / PI * q * df
+
+
 # ------------------------------------------------------------------ GPU entry points
 def device_count():
     return _lib.load().negf_device_count()
@@ -342,6 +365,20 @@ class NegfDevice:
                    "negf_transmission_batch")
         self.last_status = st
         return (tau, jobs.reshape(len(kx), len(E))) if per_job else tau
+
+    def neq_density_matrix(self, E, w, kpoints=None, out=None):
+        """Bias-window part of the non-equilibrium density (extension):
+        Density_Matrix += sum_k w_k sum_n w_n [G Gamma_L G^dagger]_kk(E_n + i eta), with the
+        nodes and weights of bias_window."""
+        EE = np.ascontiguousarray(E, dtype=np.float64)
+        ww = np.ascontiguousarray(w, dtype=np.complex128)
+        kx, ky, kw = self._kp(kpoints)
+        if out is None:
+            out = np.zeros((self.N, self.N, self.N_z), dtype=np.complex128, order="F")
+        self.last_status = check(self._lib.negf_neq_density_batch(
+            self._h, dptr(EE), dptr(ww), len(EE), dptr(kx), dptr(ky), dptr(kw), len(kx), dptr(out)),
+            "negf_neq_density_batch")
+        return out
 
     def density_matrix(self, z, w, kpoints=None, out=None):
         """Density_Matrix(N, N, N_z) += sum_k sum_n w_k w_n G_kk(z_n + i eta)

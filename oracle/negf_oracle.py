@@ -447,6 +447,80 @@ def density_matrix(sys: System, N_circle: int, N_line: int, min_V: float, quadra
     return DM
 
 
+# ---------------------------------------------------------------- bias (extension)
+# Non-equilibrium density for VDS != 0.  No reference counterpart: the reference rejects
+# VDS != 0 (src/main.f90:129-132).  Convention: mu_L = MU, mu_R = MU - VDS.  The left
+# lead is attached to slice 1 with gamma^L_i = sin(Q^L_i Delta) / Delta^2 (Q^L from V_L,
+# real energy, closed channels 0 -- the trace's formula, src/grid.f90:355-360).
+def gamma_lead(energy: float, V_lead: float, nx, ny, Lx, Ly, Lz, N_z, kx, ky):
+    delta = Lz / N_z
+    Q = np.sqrt(np.maximum(2 * (energy - V_lead) - kinetic(nx, ny, Lx, Ly, kx, ky), 0.0))
+    return np.sin(Q * delta) / delta ** 2
+
+
+def neq_spectral_blocks(sys: System, energy: float, ik: int, lead: str = "L"):
+    """Diagonal blocks of A_L = G Gamma_L G^dagger (lead "L": Gamma on slice 1 from V_L)
+    or A_R (lead "R": Gamma on slice N_z from V_R), physical units (G = 2 Delta^2 times
+    the inverse of the scaled operator), by dense inversion of tridiag(I, D_k, I):
+    (N_z, N, N)."""
+    a = sys.atomic
+    D = _blocks(sys, ik, energy + 1j * a.ETA)
+    N_z, N = sys.N_z, sys.N
+    G = np.linalg.inv(assemble_dense(D)) * 2.0 * (a.LZ / N_z) ** 2
+    V = a.V_L if lead == "L" else a.V_R
+    gam = gamma_lead(energy, V, sys.nx, sys.ny, a.LX, a.LY, a.LZ, N_z, sys.kx[ik], sys.ky[ik])
+    col = G[:, :N] if lead == "L" else G[:, (N_z - 1) * N:]     # G_{k,1} or G_{k,N_z}
+    out = np.empty((N_z, N, N), dtype=np.complex128)
+    for k in range(N_z):
+        Gk1 = col[k * N:(k + 1) * N]
+        out[k] = (Gk1 * gam[None, :]) @ Gk1.conj().T
+    return out
+
+
+def bias_window(atomic: Atomic, VDS: float, n_nodes: int, quadrature: str = "simpson"):
+    """Real-axis nodes and weights of the bias-window integral
+        Density_Matrix += i (1/pi) integral (f_L - f_R) A_L dE,  mu_L = MU, mu_R = MU - VDS
+    over [min(mu) - GAP, max(mu) + GAP] (GAP in Hartree as t_parameters holds it).  The
+    i makes Im(Density_Matrix), which the density reads, carry (1/pi) integral (f_L - f_R) A_L
+    (2 for spin times 1/(2 pi)).  Returns (E, w) with w complex."""
+    muL, muR = atomic.MU, atomic.MU - VDS
+    lo, hi = min(muL, muR) - atomic.GAP, max(muL, muR) + atomic.GAP
+    if quadrature == "gauss-legendre":
+        x, g = np.polynomial.legendre.leggauss(n_nodes)
+        E = (hi + lo) / 2 + (hi - lo) / 2 * x
+        q = (hi - lo) / 2 * g
+    else:
+        E = np.array([value_simpson(i, n_nodes, lo, hi) for i in range(1, n_nodes + 1)])
+        q = np.array([coefficient_simpson(i, n_nodes, lo, hi) for i in range(1, n_nodes + 1)])
+    df = np.array([fermi_func(e, muL, atomic.TEMPERATURE) - fermi_func(e, muR, atomic.TEMPERATURE) for e in E])
+    return E, 1j / PI * q * df
+
+
+def density_matrix_bias(sys: System, N_circle: int, N_line: int, min_V: float, VDS: float, n_window: int,
+                        quadrature: str = "simpson"):
+    """Density_Matrix for VDS != 0: the equilibrium contour with the Fermi level at
+    min(mu_L, mu_R), plus the bias window with A_L of the lead whose level is higher
+    (VDS > 0: the left lead)."""
+    a = sys.atomic
+    mu_min = min(a.MU, a.MU - VDS)
+    import dataclasses
+    a_eq = dataclasses.replace(a, MU=mu_min)
+    DM = np.zeros((sys.N_z, sys.N, sys.N), dtype=np.complex128)
+    sys_eq = dataclasses.replace(sys, atomic=a_eq)
+    for ik in range(len(sys.kx)):
+        for E, w in density_contour(a_eq, N_circle, N_line, min_V, quadrature):
+            DM += sys.kw[ik] * w * diagonal_blocks(sys_eq, E, ik)
+    if VDS == 0.0:
+        return DM
+    if VDS < 0.0:
+        raise ValueError("VDS < 0 (right lead higher) is not restated")
+    Ew, ww = bias_window(a, VDS, n_window, quadrature)
+    for ik in range(len(sys.kx)):
+        for E, w in zip(Ew, ww):
+            DM += sys.kw[ik] * w * neq_spectral_blocks(sys, float(E), ik)
+    return DM
+
+
 def ldos_energies(atomic: Atomic, N_E: int):
     """src/negf.f90:203-204: last point excluded."""
     e0, e1 = atomic.LDOS_ENERGY_GRID

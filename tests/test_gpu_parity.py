@@ -246,6 +246,26 @@ def test_ldos_small(ng):
         assert rel_err(got[:, :, :, i], np.moveaxis(ref, 0, 2)) < RTOL
 
 
+def test_neq_density_small(ng):
+    """Bias-window part of the non-equilibrium density (extension) against the oracle's
+    dense-inversion restatement, two k points, Simpson and Gauss-Legendre windows."""
+    s, V = small_system(N_z=6, seed=23, NK=(2, 1), eta=1e-4, VL=0.01, VR=0.0)
+    a = s.atomic
+    a.MU, a.TEMPERATURE = 0.5, 300.0
+    a.GAP = 6 * O.K_B * a.TEMPERATURE
+    dev = device_for(ng, s)
+    ha = ng.Atomic(MU=a.MU, TEMPERATURE=a.TEMPERATURE, GAP=a.GAP)
+    for quad in ("simpson", "gauss-legendre"):
+        E, w = ng.bias_window(ha, 0.02, 9, quad)
+        got = dev.neq_density_matrix(E, w, (s.kx, s.ky, s.kw))
+        assert dev.last_status == 0
+        ref = np.zeros((s.N_z, s.N, s.N), dtype=np.complex128)
+        for ik in range(len(s.kx)):
+            for e, q in zip(E, w):
+                ref += s.kw[ik] * q * O.neq_spectral_blocks(s, float(e), ik, "L")
+        assert rel_err(got, np.moveaxis(ref, 0, 2)) < RTOL
+
+
 def test_pw_to_real(ng):
     L = 6.0
     nx, ny = O.planewave_basis(4.5 * (2 * np.pi / L) ** 2 / 2, L, L)
