@@ -11,7 +11,7 @@ module negf
     private
     public Equilibrium_Density, Transmission_Coefficient, Local_Density_Of_State
     public Transmission_batch, Density_batch, LDOS_batch, contour_node, negf_release, negf_precision
-    public negf_contour_rule
+    public negf_contour_rule, Bias_density_batch
 
     type(c_ptr), save :: ctx = c_null_ptr
     complex*16, allocatable, save :: key_V(:, :, :)
@@ -231,6 +231,51 @@ contains
                         "negf_density_batch")
         seconds = negf_last_kernel_seconds(ctx)
     end subroutine Density_batch
+
+    ! Bias-window part of the non-equilibrium density (extension; the reference rejects
+    ! VDS /= 0, src/main.f90:129-132), all window nodes and k points in one launch:
+    !   Density_Matrix += i/pi sum_n q_n (f_L - f_R)(E_n) [G Gamma_L G^dagger]_kk (E_n + i eta)
+    ! with mu_L = MU, mu_R = MU - VDS and E_n on [mu_R - GAP, mu_L + GAP] by the contour's
+    ! rule (Simpson or Gauss-Legendre).  The i puts the term into Im(Density_Matrix), which
+    ! the density reads (src/main.f90:347).
+    subroutine Bias_density_batch(V_reciprocal_all, nx_grid, ny_grid, N_window, atomic, kpoint, &
+                                  Density_Matrix, seconds)
+        complex*16, intent(in) :: V_reciprocal_all(:, :, :)
+        integer, intent(in) :: nx_grid(:), ny_grid(:), N_window
+        type(t_parameters), intent(in) :: atomic
+        type(t_kpointmesh), intent(in) :: kpoint(:)
+        complex*16, intent(inout) :: Density_Matrix(:, :, :)
+        real*8, intent(out) :: seconds
+        real(c_double), allocatable :: E(:), kx(:), ky(:), kw(:), gx(:), gw(:)
+        complex(c_double_complex), allocatable :: w(:)
+        real*8 :: lo, hi, q, muL, muR
+        integer :: i
+        call bind_system(V_reciprocal_all, nx_grid, ny_grid, atomic)
+        muL = atomic%MU
+        muR = atomic%MU - atomic%VDS
+        lo = muR - atomic%GAP
+        hi = muL + atomic%GAP
+        allocate(E(N_window), w(N_window), gx(N_window), gw(N_window))
+        if (negf_contour_rule == 1) call gauss_legendre(N_window, gx, gw)
+        do i = 1, N_window
+            if (negf_contour_rule == 1) then
+                E(i) = (hi + lo) / 2 + (hi - lo) / 2 * gx(i)
+                q = (hi - lo) / 2 * gw(i)
+            else
+                E(i) = value_simpson(i, N_window, lo, hi)
+                q = coefficient_simpson(i, N_window, lo, hi)
+            end if
+            w(i) = cmplx(0.D0, q / pi * (fermi_func(E(i), muL, atomic%TEMPERATURE) - &
+                         fermi_func(E(i), muR, atomic%TEMPERATURE)), kind=8)
+        end do
+        allocate(kx(size(kpoint)), ky(size(kpoint)), kw(size(kpoint)))
+        kx = kpoint(:)%kx
+        ky = kpoint(:)%ky
+        kw = kpoint(:)%weight
+        call negf_check(negf_neq_density_batch(ctx, E, w, N_window, kx, ky, kw, size(kx), Density_Matrix), &
+                        "negf_neq_density_batch")
+        seconds = negf_last_kernel_seconds(ctx)
+    end subroutine Bias_density_batch
 
     ! LDOS_Matrix for one energy index (all k points in one launch).
     subroutine LDOS_batch(i_energy, N_E, V_reciprocal_all, nx_grid, ny_grid, atomic, kpoint, LDOS_Matrix, seconds)

@@ -12,6 +12,9 @@
 !   CONTOUR_QUADRATURE = 'SIMPSON' (the reference's rule) | 'GAUSS-LEGENDRE' (density contour)
 !   FULL_WINDOW    = .false. (copy only |p|<=NGX/2, |q|<=NGY/2 of V-hat, as the reference)
 !   DUMP_FULL      = .false. (also write *_FULL files with 17 significant digits)
+!   BIAS_DENSITY   = .false. (.true. accepts VDS > 0: the density is the equilibrium contour
+!                    at mu_R = MU - VDS plus the bias window [mu_R - GAP, MU + GAP] with
+!                    G Gamma_L G^dagger; the reference rejects VDS /= 0)
 program negftrans_gpu
     use, intrinsic :: iso_c_binding
     use global
@@ -29,12 +32,13 @@ program negftrans_gpu
     integer :: N_circle = 40, N_line_per_eV = 500
     integer :: PRECISION_MODE = 0
     character(len=32) :: CONTOUR_QUADRATURE = 'SIMPSON'
-    logical :: FULL_WINDOW = .false., DUMP_FULL = .false.
+    logical :: FULL_WINDOW = .false., DUMP_FULL = .false., BIAS_DENSITY = .false.
     namelist /INPUT/ V_L, V_R, MU, ETA, TEMPERATURE, LX, LY, LZ, NGX, NGY, ENCUT, &
         N_circle, N_line_per_eV, GAP, NKX, NKY, VDS, TRANSMISSION_GRID, if_density, if_transmission, &
-        if_LDOS, LDOS_ENERGY_GRID, LDOS_GRID, PRECISION_MODE, FULL_WINDOW, DUMP_FULL, CONTOUR_QUADRATURE
+        if_LDOS, LDOS_ENERGY_GRID, LDOS_GRID, PRECISION_MODE, FULL_WINDOW, DUMP_FULL, CONTOUR_QUADRATURE, &
+        BIAS_DENSITY
 
-    type(t_parameters) :: atomic
+    type(t_parameters) :: atomic, atomic_eq
     type(t_kpointmesh), allocatable :: kpoint(:)
     type(t_transmission), allocatable :: Transmission(:)
     real*8, allocatable :: V_real(:, :, :), Density(:, :, :), LDOS(:, :, :, :)
@@ -42,9 +46,9 @@ program negftrans_gpu
     complex(c_double_complex), allocatable :: realspace(:, :, :)
     integer, allocatable :: nx_grid(:), ny_grid(:)
     integer(c_int), allocatable :: cnx(:), cny(:)
-    integer :: N_x, N_y, N_z, N, N_E, N_line, i, j, k, ii, jj, kk, ie, io, bound
+    integer :: N_x, N_y, N_z, N, N_E, N_line, N_window, i, j, k, ii, jj, kk, ie, io, bound
     real*8 :: rescale_factor, gpu_s, t_total0, t_total1, t_rtop, t_dens, t_ptor, t_trans, t_ldos, t_ldos_ptor
-    real*8 :: t0, t1
+    real*8 :: t0, t1, gpu_s2
     logical :: bad
 
     call wall(t_total0)
@@ -91,8 +95,12 @@ program negftrans_gpu
         write(16, *) "ERROR: LDOS_GRID must be specified for (if_LDOS = .true.)"
         bad = .true.
     end if
-    if (VDS /= 0.D0) then
+    if (VDS /= 0.D0 .and. .not. BIAS_DENSITY) then
         write(16, *) "ERROR: VDS must be zero in this version, please upgrade to higher version."
+        bad = .true.
+    end if
+    if (BIAS_DENSITY .and. VDS < 0.D0) then
+        write(16, *) "ERROR: BIAS_DENSITY needs VDS >= 0 (the left lead at the higher Fermi level)"
         bad = .true.
     end if
     if (bad) call stop_run()
@@ -183,8 +191,21 @@ program negftrans_gpu
         allocate(Density(N_x, N_y, N_z), Density_Matrix(N, N, N_z), realspace(N_x, N_y, N_z))
         Density_Matrix = (0.D0, 0.D0)
         call wall(t0)
-        call Density_batch(V_reciprocal_all, nx_grid, ny_grid, N_circle, N_line, minval(V_real), atomic, &
-                           kpoint, Density_Matrix, gpu_s)
+        if (BIAS_DENSITY .and. atomic%VDS > 0.D0) then
+            ! equilibrium part up to the lower level mu_R = MU - VDS, then the window
+            atomic_eq = atomic
+            atomic_eq%MU = atomic%MU - atomic%VDS
+            N_window = max(3, nint(N_line_per_eV * ((atomic%VDS + 2 * atomic%GAP) * hartree)))
+            write(16, *) "  bias window integral:", N_window
+            call Density_batch(V_reciprocal_all, nx_grid, ny_grid, N_circle, N_line, minval(V_real), &
+                               atomic_eq, kpoint, Density_Matrix, gpu_s)
+            call Bias_density_batch(V_reciprocal_all, nx_grid, ny_grid, N_window, atomic, kpoint, &
+                                    Density_Matrix, gpu_s2)
+            gpu_s = gpu_s + gpu_s2
+        else
+            call Density_batch(V_reciprocal_all, nx_grid, ny_grid, N_circle, N_line, minval(V_real), atomic, &
+                               kpoint, Density_Matrix, gpu_s)
+        end if
         call wall(t1)
         t_dens = t1 - t0
         write(16, *) "Transform Green's Function from plane wave basis to real space:"

@@ -143,3 +143,45 @@ def test_fortran_host_gauss_legendre_density(tmp_path):
     DS = O.density_real(O.density_matrix(s, CASE["N_circle"], N_line, float(V.min())), s.nx, s.ny, N_x, N_y,
                         CASE["LX"], CASE["LY"], CASE["LZ"])
     assert np.linalg.norm(DS - Dref) > 1e-8 * np.linalg.norm(Dref)
+
+
+def test_bias_density_negative_vds_rejected(tmp_path):
+    """BIAS_DENSITY (extension key) restates only VDS >= 0: the left lead at the higher level."""
+    write_case(tmp_path, {"if_density": True, "LX": 5.0, "LY": 5.0, "LZ": 5.0, "NKX": 1, "NKY": 1,
+                          "VDS": -0.1, "BIAS_DENSITY": True}, np.zeros((4, 4, 2)))
+    r = run(tmp_path)
+    assert r.returncode != 0
+    out = open(tmp_path / "OUTPUT").read()
+    assert "BIAS_DENSITY needs VDS >= 0" in out and "VDS must be zero" not in out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("quad", ["SIMPSON", "GAUSS-LEGENDRE"])
+def test_fortran_host_bias_density(tmp_path, quad):
+    """BIAS_DENSITY = .true., VDS = 0.1 eV: DENSITY against the oracle's equilibrium contour at
+    mu_R = MU - VDS plus the bias window with G Gamma_L G^dagger (dense inversion), relative
+    1e-10.  The window's node count follows N_line_per_eV over VDS + 2 GAP."""
+    N_x, N_y, N_z = 24, 28, 4
+    VDS = 0.1
+    V_eV = 0.3 * np.random.default_rng(11).standard_normal((N_x, N_y, N_z)) + 1.0
+    inp = dict(CASE, if_density=True, DUMP_FULL=True, BIAS_DENSITY=True, VDS=VDS,
+               CONTOUR_QUADRATURE=f"'{quad}'")
+    write_case(tmp_path, inp, V_eV)
+    r = run(tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr + open(tmp_path / "OUTPUT").read()
+    s, V = oracle_case(V_eV)
+    a = s.atomic
+    N_line = O.n_line(a, CASE["N_line_per_eV"])
+    x = CASE["N_line_per_eV"] * ((VDS / O.HARTREE + 2 * a.GAP) * O.HARTREE)
+    n_window = max(3, int(np.floor(x + 0.5)))
+    assert f"bias window integral:{n_window:12d}".replace(" ", "") in \
+        open(tmp_path / "OUTPUT").read().replace(" ", "")
+    q = quad.lower()
+    DM = O.density_matrix_bias(s, CASE["N_circle"], N_line, float(V.min()), VDS / O.HARTREE, n_window, q)
+    Dref = O.density_real(DM, s.nx, s.ny, N_x, N_y, CASE["LX"], CASE["LY"], CASE["LZ"])
+    Dgot = np.loadtxt(tmp_path / "DENSITY_FULL", skiprows=1).ravel().reshape((N_x, N_y, N_z), order="F")
+    assert np.linalg.norm(Dgot - Dref) <= 1e-10 * np.linalg.norm(Dref)
+    # the bias moves the density by more than the bar (the window term is present)
+    D0 = O.density_real(O.density_matrix(s, CASE["N_circle"], N_line, float(V.min()), q), s.nx, s.ny,
+                        N_x, N_y, CASE["LX"], CASE["LY"], CASE["LZ"])
+    assert np.linalg.norm(D0 - Dref) > 1e-6 * np.linalg.norm(Dref)

@@ -361,6 +361,33 @@ def test_neq_bias_window_and_zero_bias_limit():
     np.testing.assert_array_equal(DM0, O.density_matrix(s, 9, 7, -0.2))
 
 
+def test_neq_closed_right_lead_equals_equilibrium_at_mu_L():
+    """Bias extension, end to end: with the right lead closed at every window energy
+    (V_R = 50 Hartree, so Gamma_R = 0), G Gamma_L G^dagger is the whole spectral function up
+    to O(eta), and the equilibrium contour at mu_R plus the bias window must give the
+    equilibrium density at mu_L.  Compared on the anti-Hermitian part (DM - DM^H) / 2i,
+    which the real-space density reads.  GAP = 36 kT removes the Fermi-tail truncation.
+    A wrong sign of f_L - f_R, a dropped 1/pi, a wrong Gamma scale or lead slice, or a
+    missing (2 Delta^2)^2 each fail it (the window term is 1.8 % of the density here,
+    the bar 1e-5; measured 1e-6)."""
+    s, V = tiny_system(N_z=3, seed=5, eta=1e-5, V_R=50.0)
+    a = s.atomic
+    a.MU, a.TEMPERATURE = 0.9, 300.0
+    a.GAP = 36 * O.K_B * a.TEMPERATURE
+    VDS = 0.01
+
+    def ah(M):
+        return (M - np.conj(np.transpose(M, (0, 2, 1)))) / 2j
+
+    ref = ah(O.density_matrix(s, 40, 61, float(V.min()), "gauss-legendre"))
+    got = ah(O.density_matrix_bias(s, 40, 61, float(V.min()), VDS, 61, "gauss-legendre"))
+    assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
+    import dataclasses
+    s_R = dataclasses.replace(s, atomic=dataclasses.replace(a, MU=a.MU - VDS))
+    eq_R = ah(O.density_matrix(s_R, 40, 61, float(V.min()), "gauss-legendre"))
+    assert np.linalg.norm(eq_R - ref) > 1e-2 * np.linalg.norm(ref)
+
+
 def test_ldos_matrix_vs_dense():
     s, V = tiny_system(N_z=4, seed=8, eta=1e-3)
     E = 0.9
