@@ -28,6 +28,13 @@ typedef struct negf_ctx negf_ctx;
 #define NEGF_PREC_MIXED_FP32 1   /* FP32 sweep kept in HBM + FP64 iterative refinement  */
 /* V-hat window: faithful copies only |p|<=NGX/2, |q|<=NGY/2 (src/main.f90:242-246). */
 #define NEGF_FULL_WINDOW     16  /* extension: use the whole +-NGX, +-NGY window    */
+/* Extension, no reference counterpart (the reference adds exp(i q Delta) with the
+ * continuum q, src/grid.f90:284-289,323-331): lead terms of a semi-infinite flat lead on
+ * the device's own finite-difference lattice, computed on the GPU by decimation (FP64,
+ * |coupling| < 1e-14 convergence test, at most 200 doubling steps; a call fails when a
+ * channel does not converge).  Trace weights become Im(term) / Delta^2 on the
+ * lattice-open channels.  A V = 0 device then transmits exactly 1 per open channel. */
+#define NEGF_LEAD_DECIMATION 32
 
 /* Library version string and last error message (thread-local). */
 const char *negf_version(void);

@@ -860,6 +860,46 @@ __global__ __launch_bounds__(256) void k_trace(int N, const cd *G, const double 
     if (threadIdx.x == 0) *tau = red[0] + red[1] + red[2] + red[3];
 }
 
+// End-slice term of a semi-infinite flat lead on the device's own finite-difference
+// lattice, by decimation (extension; the reference adds exp(i q Delta) with the continuum
+// q, src/grid.f90:284-289,323-331).  One thread per (lead, job, basis function).  The
+// scaled lead is the chain tridiag(1, d, 1) with d = 2 Delta^2 (z - V - |G + k|^2 / 2) - 2;
+// Sancho-Rubio doubling steps in FP64,
+//     g = 1/e;  e_s -= a g b;  e -= a g b + b g a;  a <- -a g a;  b <- -b g b,
+// run from e_s = e = d, a = b = 1 until |re a| + |im a| < tol.  out = -1/e_s, the term
+// added to the end block; iters = steps taken, or -1 when max_iter was reached (an open
+// channel at eta = 0 never converges).  out and iters are [2][nj][Np] (left, then right).
+__global__ __launch_bounds__(256) void k_lead_decimation(int nj, int N, int Np, double two_d2, double VL,
+                                                         double VR, const cd *z, const int *kidx,
+                                                         const double *kinh, double tol, int max_iter,
+                                                         cd *out, int *iters) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= 2 * nj * N) return;
+    const int lead = id / (nj * N), rem = id % (nj * N), job = rem / N, i = rem % N;
+    const cd zz = z[job];
+    const double kh = kinh[(size_t)kidx[job] * Np + i];
+    const cd d = cmk(two_d2 * (zz.x - (lead ? VR : VL) - kh) - 2.0, two_d2 * zz.y);
+    cd es = d, e = d, a = cmk(1.0, 0.0), b = cmk(1.0, 0.0);
+    int done = -1;
+    for (int n = 1; n <= max_iter; ++n) {
+        const cd g = crecip(e);
+        const cd agb = cmul(cmul(a, g), b), bga = cmul(cmul(b, g), a);
+        es = csub(es, agb);
+        e = csub(csub(e, agb), bga);
+        const cd an = cmul(cmul(a, g), a), bn = cmul(cmul(b, g), b);
+        a = cmk(-an.x, -an.y);
+        b = cmk(-bn.x, -bn.y);
+        if (fabs(a.x) + fabs(a.y) < tol) {
+            done = n;
+            break;
+        }
+    }
+    const cd ge = crecip(es);
+    const size_t o = ((size_t)lead * nj + job) * Np + i;
+    out[o] = cmk(-ge.x, -ge.y);
+    iters[o] = done;
+}
+
 // ------------------------------------------------------------------ host side
 struct negf_ctx {
     int device = 0;
@@ -948,10 +988,13 @@ struct JobTables {
     std::vector<int> kidx;
 };
 
-void job_tables(const negf_ctx *c, const std::vector<std::complex<double>> &energies,
-                const std::vector<double> &realE, const double *kx, const double *ky, int nk,
-                const std::vector<int> &job_e, const std::vector<int> &job_k, bool with_gamma,
-                JobTables &t) {
+int lead_decimation(const negf_ctx *c, JobTables &t, const std::vector<double> &realE,
+                    const std::vector<int> &job_e, const std::vector<double> &kin, bool with_gamma);
+
+int job_tables(const negf_ctx *c, const std::vector<std::complex<double>> &energies,
+               const std::vector<double> &realE, const double *kx, const double *ky, int nk,
+               const std::vector<int> &job_e, const std::vector<int> &job_k, bool with_gamma,
+               JobTables &t) {
     const int Np = c->Np, N = c->N;
     const double delta = c->Lz / c->Nz;
     t.kinh.assign((size_t)nk * Np, 0.0);
@@ -995,6 +1038,53 @@ void job_tables(const negf_ctx *c, const std::vector<std::complex<double>> &ener
             }
         }
     }
+    if (c->flags & NEGF_LEAD_DECIMATION) return lead_decimation(c, t, realE, job_e, kin, with_gamma);
+    return 0;
+}
+
+// NEGF_LEAD_DECIMATION: replace the closed-form lead terms by k_lead_decimation's and
+// the trace weights by Im(term) / Delta^2 on the lattice-open channels
+// (0 < Delta^2 (E - V - |G + k|^2 / 2) < 2 at real E), 0 elsewhere.
+int lead_decimation(const negf_ctx *c, JobTables &t, const std::vector<double> &realE,
+                    const std::vector<int> &job_e, const std::vector<double> &kin, bool with_gamma) {
+    const int Np = c->Np, N = c->N, nj = (int)t.z.size();
+    const int nk = (int)(t.kinh.size() / Np);
+    const double delta = c->Lz / c->Nz;
+    Temps tmp;
+    cd *d_z, *d_out;
+    int *d_k, *d_it;
+    double *d_kin;
+    int st;
+    const size_t n = (size_t)2 * nj * Np;
+    if ((st = tmp.upload(&d_z, t.z.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_k, t.kidx.data(), nj, c->stream)) ||
+        (st = tmp.upload(&d_kin, t.kinh.data(), (size_t)nk * Np, c->stream)) || (st = tmp.alloc(&d_out, n)) ||
+        (st = tmp.alloc(&d_it, n)))
+        return st;
+    const int threads = 2 * nj * N;
+    hipLaunchKernelGGL(k_lead_decimation, dim3((threads + 255) / 256), dim3(256), 0, c->stream, nj, N, Np,
+                       2.0 * delta * delta, c->VL, c->VR, d_z, d_k, d_kin, 1e-14, 200, d_out, d_it);
+    HIPCHK(hipGetLastError());
+    std::vector<cd> out(n);
+    std::vector<int> it(n);
+    HIPCHK(hipMemcpyAsync(out.data(), d_out, n * sizeof(cd), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(it.data(), d_it, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < N; ++i) {
+            const size_t oL = (size_t)j * Np + i, oR = (size_t)(nj + j) * Np + i;
+            if (it[oL] < 0 || it[oR] < 0)
+                return fail(-21, "lead decimation did not converge in 200 steps (eta = 0 on an open channel?)");
+            t.sigL[oL] = out[oL];
+            t.sigR[oL] = out[oR];
+            if (with_gamma) {
+                const double Er = realE[job_e[j]], kn = kin[(size_t)t.kidx[j] * N + i];
+                const double xL = delta * delta * (Er - c->VL - kn / 2), xR = delta * delta * (Er - c->VR - kn / 2);
+                t.gamL[oL] = (xL > 0 && xL < 2) ? out[oL].y / (delta * delta) : 0.0;
+                t.gamR[oL] = (xR > 0 && xR < 2) ? out[oR].y / (delta * delta) : 0.0;
+            }
+        }
+    return 0;
 }
 
 int check_ctx(const negf_ctx *c) {
@@ -1178,7 +1268,7 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
         jk[j] = j / nE;
     }
     JobTables t;
-    job_tables(c, energies, realE, kx, ky, nk, je, jk, true, t);
+    if ((st = job_tables(c, energies, realE, kx, ky, nk, je, jk, true, t))) return st;
     // open-channel lists per job
     std::vector<int> off(nj), nL(nj), nR(nj), Lrow, Rcol;
     std::vector<double> gLv, gRv;
@@ -1367,7 +1457,7 @@ static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &e
     std::vector<double> realE(energies.size());
     for (size_t i = 0; i < energies.size(); ++i) realE[i] = energies[i].real();
     JobTables t;
-    job_tables(c, energies, realE, kx, ky, nk, je, jk, false, t);
+    if ((st = job_tables(c, energies, realE, kx, ky, nk, je, jk, false, t))) return st;
     Temps tmp;
     DiagArgs a;
     a.sv = sysview(c);
@@ -1436,14 +1526,14 @@ static int blocks_common(negf_ctx *c, int mode, std::complex<double> E, double k
     if (st) return st;
     if (!out) return fail(-2, "out is null");
     HIPCHK(hipSetDevice(c->device));
-    std::vector<std::complex<double>> en{E};
+    // job_tables adds +i*eta; the block builders take the energy as given
+    std::vector<std::complex<double>> en{E - std::complex<double>(0.0, c->eta)};
     std::vector<double> re{E.real()};
     std::vector<int> je{0}, jk{0};
     JobTables t;
-    job_tables(c, en, re, &kx, &ky, 1, je, jk, false, t);
-    // job_tables adds +i*eta; the block builders take the energy as given
+    if ((st = job_tables(c, en, re, &kx, &ky, 1, je, jk, false, t))) return st;
     t.z[0] = make_double2(E.real(), E.imag());
-    {
+    if (!(c->flags & NEGF_LEAD_DECIMATION)) {
         const double delta = c->Lz / c->Nz;
         for (int i = 0; i < c->N; ++i) {
             const double gx = 2 * c->nx[i] * PI / c->Lx + kx, gy = 2 * c->ny[i] * PI / c->Ly + ky;
@@ -1610,7 +1700,7 @@ int negf_neq_density_batch(negf_ctx *c, const double *E, const double *w, int nE
         hw[j] = make_double2(wj.real(), wj.imag());
     }
     JobTables t;
-    job_tables(c, energies, realE, kx, ky, nk, je, jk, true, t);
+    if ((st = job_tables(c, energies, realE, kx, ky, nk, je, jk, true, t))) return st;
     std::vector<int> off(nj), nL(nj), Lcol;
     std::vector<double> gLv;
     int nLmax = 1;

@@ -11,13 +11,16 @@ module negf
     private
     public Equilibrium_Density, Transmission_Coefficient, Local_Density_Of_State
     public Transmission_batch, Density_batch, LDOS_batch, contour_node, negf_release, negf_precision
-    public negf_contour_rule, Bias_density_batch
+    public negf_contour_rule, Bias_density_batch, negf_lead
 
     type(c_ptr), save :: ctx = c_null_ptr
     complex*16, allocatable, save :: key_V(:, :, :)
     integer, allocatable, save :: key_nx(:), key_ny(:)
     real*8, save :: key_p(6) = 0.D0
+    integer(c_int), save :: key_flags = -1
     integer(c_int), save :: negf_precision = NEGF_PREC_FP64
+    ! NEGF_LEAD_DECIMATION or 0 (the reference's closed-form lead terms)
+    integer(c_int), save :: negf_lead = 0
     ! 0: the reference's Simpson rule; 1: Gauss-Legendre nodes on the arc and the line
     integer, save :: negf_contour_rule = 0
 contains
@@ -30,6 +33,9 @@ contains
         integer(c_int), allocatable :: nx(:), ny(:)
         complex(c_double_complex), allocatable :: V(:, :, :)
         p = [atomic%LX, atomic%LY, atomic%LZ, atomic%V_L, atomic%V_R, atomic%ETA]
+        if (c_associated(ctx)) then
+            if (key_flags /= ior(negf_precision, negf_lead)) call negf_release()
+        end if
         if (c_associated(ctx)) then
             if (all(shape(key_V) == shape(V_reciprocal_all)) .and. size(key_nx) == size(nx_grid)) then
                 if (all(key_V == V_reciprocal_all) .and. all(key_nx == nx_grid) .and. &
@@ -44,11 +50,12 @@ contains
         V = V_reciprocal_all
         call negf_check(negf_ctx_create(ctx, size(nx), size(V, 3), nx, ny, atomic%LX, atomic%LY, &
                         atomic%LZ, atomic%V_L, atomic%V_R, atomic%ETA, V, (size(V, 1) - 1) / 2, &
-                        (size(V, 2) - 1) / 2, negf_precision), "negf_ctx_create")
+                        (size(V, 2) - 1) / 2, ior(negf_precision, negf_lead)), "negf_ctx_create")
         key_V = V_reciprocal_all
         key_nx = nx_grid
         key_ny = ny_grid
         key_p = p
+        key_flags = ior(negf_precision, negf_lead)
     end subroutine bind_system
 
     subroutine negf_release()

@@ -7,7 +7,7 @@ module negf_gpu_c
     public
 
     integer(c_int), parameter :: NEGF_PREC_FP64 = 0, NEGF_PREC_MIXED_FP32 = 1
-    integer(c_int), parameter :: NEGF_FULL_WINDOW = 16
+    integer(c_int), parameter :: NEGF_FULL_WINDOW = 16, NEGF_LEAD_DECIMATION = 32
     integer(c_int), parameter :: NEGF_MODE_D = 1, NEGF_MODE_F = 2, NEGF_MODE_L = 4
 
     interface

@@ -12,6 +12,8 @@
 !   CONTOUR_QUADRATURE = 'SIMPSON' (the reference's rule) | 'GAUSS-LEGENDRE' (density contour)
 !   FULL_WINDOW    = .false. (copy only |p|<=NGX/2, |q|<=NGY/2 of V-hat, as the reference)
 !   DUMP_FULL      = .false. (also write *_FULL files with 17 significant digits)
+!   LEAD_SELF_ENERGY = 'CLOSED-FORM' (the reference's exp(i q Delta)) | 'DECIMATION' (lattice
+!                    lead, decimated on the GPU; a V = 0 device transmits 1 per open channel)
 !   BIAS_DENSITY   = .false. (.true. accepts VDS > 0: the density is the equilibrium contour
 !                    at mu_R = MU - VDS plus the bias window [mu_R - GAP, MU + GAP] with
 !                    G Gamma_L G^dagger; the reference rejects VDS /= 0)
@@ -31,12 +33,12 @@ program negftrans_gpu
     integer :: LDOS_GRID(3) = -10, NGX = 0, NGY = 0, NKX = 0, NKY = 0
     integer :: N_circle = 40, N_line_per_eV = 500
     integer :: PRECISION_MODE = 0
-    character(len=32) :: CONTOUR_QUADRATURE = 'SIMPSON'
+    character(len=32) :: CONTOUR_QUADRATURE = 'SIMPSON', LEAD_SELF_ENERGY = 'CLOSED-FORM'
     logical :: FULL_WINDOW = .false., DUMP_FULL = .false., BIAS_DENSITY = .false.
     namelist /INPUT/ V_L, V_R, MU, ETA, TEMPERATURE, LX, LY, LZ, NGX, NGY, ENCUT, &
         N_circle, N_line_per_eV, GAP, NKX, NKY, VDS, TRANSMISSION_GRID, if_density, if_transmission, &
         if_LDOS, LDOS_ENERGY_GRID, LDOS_GRID, PRECISION_MODE, FULL_WINDOW, DUMP_FULL, CONTOUR_QUADRATURE, &
-        BIAS_DENSITY
+        BIAS_DENSITY, LEAD_SELF_ENERGY
 
     type(t_parameters) :: atomic, atomic_eq
     type(t_kpointmesh), allocatable :: kpoint(:)
@@ -169,7 +171,16 @@ program negftrans_gpu
         negf_contour_rule = 1
     case default
         write(16, *) "ERROR: CONTOUR_QUADRATURE must be SIMPSON or GAUSS-LEGENDRE"
-        stop 1
+        call stop_run()
+    end select
+    select case (trim(adjustl(LEAD_SELF_ENERGY)))
+    case ('CLOSED-FORM', 'closed-form')
+        negf_lead = 0
+    case ('DECIMATION', 'decimation')
+        negf_lead = NEGF_LEAD_DECIMATION
+    case default
+        write(16, *) "ERROR: LEAD_SELF_ENERGY must be CLOSED-FORM or DECIMATION"
+        call stop_run()
     end select
 
     ! V_reciprocal_all for every slice on the GPU, window as the reference copies it

@@ -26,6 +26,7 @@ PI = 3.141592653589793
 MODE_D, MODE_F, MODE_L = 1, 2, 4
 PREC_FP64, PREC_MIXED_FP32 = 0, 1
 FULL_WINDOW = 16
+LEAD_DECIMATION = 32     # extension: lattice lead terms by decimation on the GPU
 
 
 # ------------------------------------------------------------------ parameters

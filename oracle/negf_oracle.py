@@ -192,21 +192,75 @@ def lead_q(energy: complex, V_lead: float, nx, ny, Lx, Ly, kx, ky):
     return np.sqrt(2 * (energy - V_lead) - kinetic(nx, ny, Lx, Ly, kx, ky) + 0j)
 
 
-def e_minus_h(H, energy: complex, nx, ny, Lx, Ly, Lz, kx, ky, V_L, V_R):
+def lead_decimation(energy: complex, V_lead: float, nx, ny, Lx, Ly, delta, kx, ky,
+                    tol: float = 1e-14, max_iter: int = 200):
+    """Extension (no reference counterpart; the reference adds exp(i q Delta) with the
+    continuum q, src/grid.f90:284-289,323-331): the end-slice term of a semi-infinite flat
+    lead on the same finite-difference lattice as the device, by Sancho-Rubio decimation.
+
+    The scaled lead is the chain tridiag(1, d_i, 1) per basis function i, with
+    d_i = 2 Delta^2 (E - V_lead - |G_i + k|^2 / 2) - 2.  Doubling step, all scalars:
+        g = 1 / e;  e_s -= a g b;  e -= a g b + b g a;  a <- -a g a;  b <- -b g b
+    from e_s = e = d, a = b = 1, until |re a| + |im a| < tol.  The surface Green's
+    function is 1 / e_s and the device's end block gets -1 / e_s (for an open channel
+    exp(i k Delta) with cos(k Delta) = 1 - Delta^2 (E - V_lead - |G + k|^2 / 2)).
+    Returns (term (N,), iterations (N,); -1 where max_iter was reached)."""
+    d = 2 * delta ** 2 * (energy - V_lead - kinetic(nx, ny, Lx, Ly, kx, ky) / 2) - 2 + 0j
+    e_s, e = d.copy(), d.copy()
+    a, b = np.ones_like(d), np.ones_like(d)
+    it = np.full(d.shape, -1)
+    done = np.zeros(d.shape, dtype=bool)
+    for n in range(1, max_iter + 1):
+        g = 1.0 / e
+        agb, bga = a * g * b, b * g * a
+        act = ~done
+        e_s = np.where(act, e_s - agb, e_s)
+        e = np.where(act, e - agb - bga, e)
+        a, b = np.where(act, -a * g * a, a), np.where(act, -b * g * b, b)
+        new = act & (np.abs(a.real) + np.abs(a.imag) < tol)
+        it[new] = n
+        done |= new
+        if done.all():
+            break
+    return -1.0 / e_s, it
+
+
+def lattice_open(energy: float, V_lead: float, nx, ny, Lx, Ly, delta, kx, ky):
+    """Open channels of the lattice lead at real energy: 0 < Delta^2 (E - V - |G+k|^2/2) < 2."""
+    x = delta ** 2 * (energy - V_lead - kinetic(nx, ny, Lx, Ly, kx, ky) / 2)
+    return (x > 0) & (x < 2)
+
+
+def lead_terms(energy: complex, nx, ny, Lx, Ly, Lz, N_z, kx, ky, V_L, V_R, lead: str = "closed-form"):
+    """The two end-slice terms added to D_1 and D_{N_z}."""
+    delta = Lz / N_z
+    if lead == "decimation":
+        sL, itL = lead_decimation(energy, V_L, nx, ny, Lx, Ly, delta, kx, ky)
+        sR, itR = lead_decimation(energy, V_R, nx, ny, Lx, Ly, delta, kx, ky)
+        if (itL < 0).any() or (itR < 0).any():
+            raise ValueError("lead decimation did not converge (eta = 0?)")
+        return sL, sR
+    if lead != "closed-form":
+        raise ValueError(f"unknown lead {lead!r}")
+    return (np.exp(1j * lead_q(energy, V_L, nx, ny, Lx, Ly, kx, ky) * delta),
+            np.exp(1j * lead_q(energy, V_R, nx, ny, Lx, Ly, kx, ky) * delta))
+
+
+def e_minus_h(H, energy: complex, nx, ny, Lx, Ly, Lz, kx, ky, V_L, V_R, lead: str = "closed-form"):
     """E_minus_H_construction (src/grid.f90:264-333).  H is (N_z, N, N).
     D_k = 2 Delta^2 (E I - H_k) - 2 I, plus exp(i q Delta) on the first and last
-    slice diagonals (closed-form lead self-energy)."""
+    slice diagonals (closed-form lead self-energy; lead="decimation" adds the lattice
+    lead's term from lead_decimation instead)."""
     N_z, N, _ = H.shape
     delta = Lz / N_z
-    QR = lead_q(energy, V_L, nx, ny, Lx, Ly, kx, ky)
-    QT = lead_q(energy, V_R, nx, ny, Lx, Ly, kx, ky)
+    sL, sR = lead_terms(energy, nx, ny, Lx, Ly, Lz, N_z, kx, ky, V_L, V_R, lead)
     idx = np.arange(N)
     D = -H
     D[:, idx, idx] += energy
     D *= (2 * delta ** 2)
     D[:, idx, idx] -= 2.0
-    D[0, idx, idx] += np.exp(1j * QR * delta)
-    D[N_z - 1, idx, idx] += np.exp(1j * QT * delta)
+    D[0, idx, idx] += sL
+    D[N_z - 1, idx, idx] += sR
     return D
 
 
@@ -281,16 +335,32 @@ def rgf_first_column_last_slice(G_inv_D: np.ndarray) -> np.ndarray:
 
 
 # ---------------------------------------------------------------- stage 4: trace
-def transmission_solver(G_last, energy: float, nx, ny, Lx, Ly, Lz, N_z, kx, ky, V_L, V_R):
-    """Transmission_solver (src/grid.f90:335-385) applied to first_column(:, :, N_z)
-    (already rescaled by 2 Delta^2).  Real energy, no eta; closed channels give 0.
-    tau = sum_ij gammaL_i gammaR_j |G(i, j)|^2 (the reference's L/R placement)."""
+def lead_gammas(energy: float, nx, ny, Lx, Ly, Lz, N_z, kx, ky, V_L, V_R, lead: str = "closed-form",
+                eta: float = 0.0):
+    """Trace weights of the two leads.  Closed form (src/grid.f90:355-360): sin(Q Delta) /
+    Delta^2 at real E, closed channels 0.  Decimation (extension): Im(term) / Delta^2 of
+    the decimated end-slice term at E + i eta on the lattice-open channels, 0 elsewhere
+    (Gamma = i (Sigma - Sigma^dagger) with Sigma = -term / (2 Delta^2))."""
     delta = Lz / N_z
+    if lead == "decimation":
+        sL, sR = lead_terms(energy + 1j * eta, nx, ny, Lx, Ly, Lz, N_z, kx, ky, V_L, V_R, lead)
+        oL = lattice_open(energy, V_L, nx, ny, Lx, Ly, delta, kx, ky)
+        oR = lattice_open(energy, V_R, nx, ny, Lx, Ly, delta, kx, ky)
+        return np.where(oL, sL.imag, 0.0) / delta ** 2, np.where(oR, sR.imag, 0.0) / delta ** 2
     kin = kinetic(nx, ny, Lx, Ly, kx, ky)
     QR = np.sqrt(np.maximum(2 * (energy - V_L) - kin, 0.0))
     QT = np.sqrt(np.maximum(2 * (energy - V_R) - kin, 0.0))
-    A = (np.sin(QR * delta) / delta ** 2)[:, None] * G_last
-    B = (np.sin(QT * delta) / delta ** 2)[:, None] * np.conj(G_last.T)
+    return np.sin(QR * delta) / delta ** 2, np.sin(QT * delta) / delta ** 2
+
+
+def transmission_solver(G_last, energy: float, nx, ny, Lx, Ly, Lz, N_z, kx, ky, V_L, V_R,
+                        lead: str = "closed-form", eta: float = 0.0):
+    """Transmission_solver (src/grid.f90:335-385) applied to first_column(:, :, N_z)
+    (already rescaled by 2 Delta^2).  Real energy, no eta; closed channels give 0.
+    tau = sum_ij gammaL_i gammaR_j |G(i, j)|^2 (the reference's L/R placement)."""
+    gL, gR = lead_gammas(energy, nx, ny, Lx, Ly, Lz, N_z, kx, ky, V_L, V_R, lead, eta)
+    A = gL[:, None] * G_last
+    B = gR[:, None] * np.conj(G_last.T)
     s = np.sum(A * B.T)
     return float(s.real)
 
@@ -340,6 +410,7 @@ class System:
     kx: np.ndarray = field(default_factory=lambda: np.zeros(1))
     ky: np.ndarray = field(default_factory=lambda: np.zeros(1))
     kw: np.ndarray = field(default_factory=lambda: np.ones(1))
+    lead: str = "closed-form"          # or "decimation" (extension)
 
     @property
     def N(self):
@@ -354,7 +425,7 @@ def _blocks(sys: System, ik: int, energy: complex):
     a = sys.atomic
     H = hamiltonian(sys.nx, sys.ny, a.LX, a.LY, sys.kx[ik], sys.ky[ik], sys.Vrec, sys.NGX, sys.NGY)
     return e_minus_h(H, energy, sys.nx, sys.ny, a.LX, a.LY, a.LZ, sys.kx[ik], sys.ky[ik],
-                     a.V_L, a.V_R)
+                     a.V_L, a.V_R, sys.lead)
 
 
 def transmission_energies(atomic: Atomic, N_E: int):
@@ -375,7 +446,7 @@ def transmission_tau(sys: System, energy: float, ik: int, faithful_solver: bool 
         G = rgf_first_column_last_slice(D)
     G = G * 2.0 * (a.LZ / sys.N_z) ** 2
     return transmission_solver(G, energy, sys.nx, sys.ny, a.LX, a.LY, a.LZ, sys.N_z,
-                               sys.kx[ik], sys.ky[ik], a.V_L, a.V_R)
+                               sys.kx[ik], sys.ky[ik], a.V_L, a.V_R, sys.lead, a.ETA)
 
 
 def transmission(sys: System, energies, faithful_solver: bool = True):
@@ -467,8 +538,9 @@ def neq_spectral_blocks(sys: System, energy: float, ik: int, lead: str = "L"):
     D = _blocks(sys, ik, energy + 1j * a.ETA)
     N_z, N = sys.N_z, sys.N
     G = np.linalg.inv(assemble_dense(D)) * 2.0 * (a.LZ / N_z) ** 2
-    V = a.V_L if lead == "L" else a.V_R
-    gam = gamma_lead(energy, V, sys.nx, sys.ny, a.LX, a.LY, a.LZ, N_z, sys.kx[ik], sys.ky[ik])
+    gL, gR = lead_gammas(energy, sys.nx, sys.ny, a.LX, a.LY, a.LZ, N_z, sys.kx[ik], sys.ky[ik],
+                         a.V_L, a.V_R, sys.lead, a.ETA)
+    gam = gL if lead == "L" else gR
     col = G[:, :N] if lead == "L" else G[:, (N_z - 1) * N:]     # G_{k,1} or G_{k,N_z}
     out = np.empty((N_z, N, N), dtype=np.complex128)
     for k in range(N_z):

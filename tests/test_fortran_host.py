@@ -185,3 +185,37 @@ def test_fortran_host_bias_density(tmp_path, quad):
     D0 = O.density_real(O.density_matrix(s, CASE["N_circle"], N_line, float(V.min()), q), s.nx, s.ny,
                         N_x, N_y, CASE["LX"], CASE["LY"], CASE["LZ"])
     assert np.linalg.norm(D0 - Dref) > 1e-6 * np.linalg.norm(Dref)
+
+
+def test_lead_key_rejected(tmp_path):
+    write_case(tmp_path, {"if_transmission": True, "LX": 5.0, "LY": 5.0, "LZ": 5.0, "NKX": 1, "NKY": 1,
+                          "TRANSMISSION_GRID": [1.0, 2.0, 4], "LEAD_SELF_ENERGY": "'SURFACE'"},
+               np.zeros((4, 4, 2)))
+    r = run(tmp_path)
+    assert r.returncode != 0
+    out = open(tmp_path / "OUTPUT").read()
+    assert "LEAD_SELF_ENERGY must be CLOSED-FORM or DECIMATION" in out and "EXIT" in out
+
+
+@pytest.mark.gpu
+def test_fortran_host_decimation_leads(tmp_path):
+    """LEAD_SELF_ENERGY = 'DECIMATION' (extension key): TRANSMISSION and DENSITY against the
+    oracle with the decimated lattice lead, relative 1e-10."""
+    import dataclasses
+    N_x, N_y, N_z = 24, 28, 5
+    V_eV = 0.3 * np.random.default_rng(13).standard_normal((N_x, N_y, N_z)) + 1.0
+    inp = dict(CASE, if_transmission=True, if_density=True, DUMP_FULL=True, LEAD_SELF_ENERGY="'DECIMATION'")
+    write_case(tmp_path, inp, V_eV)
+    r = run(tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr + open(tmp_path / "OUTPUT").read()
+    s, V = oracle_case(V_eV)
+    s = dataclasses.replace(s, lead="decimation")
+    E = O.transmission_energies(s.atomic, 9)
+    Tref = O.transmission(s, E)
+    got = np.loadtxt(tmp_path / "TRANSMISSION_FULL")
+    np.testing.assert_allclose(got[:, 1], Tref, rtol=1e-10, atol=1e-13)
+    N_line = O.n_line(s.atomic, CASE["N_line_per_eV"])
+    DM = O.density_matrix(s, CASE["N_circle"], N_line, float(V.min()))
+    Dref = O.density_real(DM, s.nx, s.ny, N_x, N_y, CASE["LX"], CASE["LY"], CASE["LZ"])
+    Dgot = np.loadtxt(tmp_path / "DENSITY_FULL", skiprows=1).ravel().reshape((N_x, N_y, N_z), order="F")
+    assert np.linalg.norm(Dgot - Dref) <= 1e-10 * np.linalg.norm(Dref)

@@ -420,3 +420,47 @@ def test_ldos_grid_converter():
     assert O.ldos_grid_converter(3, 3, 0.5, 2.0) == (1, 2.0)
     assert O.ldos_grid_converter(3, 4, 0.5, 2.0) == (1, 0.0)
     assert O.ldos_size(0, 7) == 7 and O.ldos_size(-1, 7) == 1 and O.ldos_size(2, 7) == 1
+
+
+def test_lead_decimation_matches_lattice_root():
+    """Decimated end-slice term (extension) against the analytic surface Green's function of
+    the chain tridiag(1, d, 1): g_s solves g^2 - d g + 1 = 0 with |g_s| < 1 (the decaying,
+    retarded root), term = -g_s.  Open, closed and above-band channels, eta > 0."""
+    s, V = tiny_system(N_z=6, seed=3, eta=1e-7)
+    a = s.atomic
+    delta = a.LZ / s.N_z
+    seen = np.zeros(3, dtype=bool)          # below band, open, above band
+    for E, Vl in ((0.3, 0.0), (1.7, 0.2), (2.5, -0.1)):
+        term, it = O.lead_decimation(E + 1j * a.ETA, Vl, s.nx, s.ny, a.LX, a.LY, delta, 0.0, 0.0)
+        kin = O.kinetic(s.nx, s.ny, a.LX, a.LY, 0.0, 0.0)
+        d = 2 * delta ** 2 * (E + 1j * a.ETA - Vl - kin / 2) - 2
+        r = np.sqrt(d * d - 4 + 0j)
+        g1, g2 = (d - r) / 2, (d + r) / 2
+        gs = np.where(np.abs(g1) < np.abs(g2), g1, g2)
+        assert (it > 0).all() and it.max() < 200
+        np.testing.assert_allclose(term, -gs, rtol=0, atol=1e-11)
+        # open channels: |term| = 1 up to eta, and the lattice dispersion holds
+        op = O.lattice_open(E, Vl, s.nx, s.ny, a.LX, a.LY, delta, 0.0, 0.0)
+        x = delta ** 2 * (E - Vl - kin / 2)
+        seen |= [(x <= 0).any(), op.any(), (x >= 2).any()]
+        np.testing.assert_allclose(np.abs(term[op]), 1.0, atol=1e-5)
+        np.testing.assert_allclose(term[op].real, 1 - delta ** 2 * (E - Vl - kin[op] / 2), atol=1e-5)
+        assert (np.abs(term[~op]) < 1).all()
+    assert seen.all()
+
+
+def test_lead_decimation_uniform_wire_transmits_one_per_open_channel():
+    """V = 0 everywhere: with the lattice lead, every open channel transmits 1 (the device
+    is the lead continued).  The reference's continuum closed form gives less (mismatch
+    between the continuum q and the lattice dispersion), which the second check shows."""
+    import dataclasses
+    s, V = tiny_system(N_z=6, seed=3, eta=1e-7)
+    s.Vrec[:] = 0
+    a = s.atomic
+    sd = dataclasses.replace(s, lead="decimation")
+    delta = a.LZ / s.N_z
+    for E in (0.3, 0.9, 1.7, 2.5):
+        n_open = int(np.sum(O.lattice_open(E, 0.0, s.nx, s.ny, a.LX, a.LY, delta, 0.0, 0.0)))
+        T = O.transmission(sd, [E])[0]
+        assert abs(T - n_open) <= 1e-5 * max(n_open, 1)
+        assert abs(O.transmission(s, [E])[0] - n_open) > 4e-4
