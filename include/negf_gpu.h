@@ -183,6 +183,38 @@ double negf_last_kernel_seconds(negf_ctx *ctx);
  * path (their results are FP64 results).  0 for FP64 contexts. */
 int negf_last_fallbacks(const negf_ctx *ctx);
 
+/*
+ * Extension, no reference counterpart: replace V_reciprocal_all of a context (same NGX,
+ * NGY, N_z), e.g. one step of a self-consistent potential loop.  Synchronous.
+ */
+int negf_ctx_set_potential(negf_ctx *ctx, const double *vrec);
+
+/*
+ * Extension, no reference counterpart (the reference runs one device per program):
+ * uneven batches.  A negf_multi holds FP64 contexts of one GPU with any N and N_z.
+ * negf_multi_transmission adds, for every device d and energy e,
+ *   tau[d][e] += sum_k kw[d][k] tau(E[d][e], kx[d][k], ky[d][k])
+ * exactly as negf_transmission_batch on that context, in one hipGraph: contexts of the
+ * same padded size share one launch (jobs handed to workgroups longest-N_z first), and
+ * the launches of different sizes run concurrently on forked streams.  The inversion
+ * variant of each padded size (matrix-core two-lanes-per-row or VALU one-thread-per-row
+ * Gauss-Jordan) is chosen by timing both on the first call; NEGF_MULTI_VARIANT=mfma|valu
+ * forces one.  Returns 0, the first zero-pivot slice (> 0), or < 0.
+ */
+typedef struct negf_multi negf_multi;
+int negf_multi_create(negf_multi **out, int ndev, negf_ctx *const *ctxs);
+int negf_multi_transmission(negf_multi *m, const double *const *E, const int *nE, const double *const *kx,
+                            const double *const *ky, const double *const *kw, const int *nk,
+                            double *const *tau);
+/* Launch the graph of the last negf_multi_transmission again (same jobs; the contexts'
+ * potentials may have been replaced) and accumulate into tau the same way. */
+int negf_multi_replay(negf_multi *m, double *const *tau);
+/* Device time of the last negf_multi_transmission or replay (HIP events), seconds. */
+double negf_multi_last_seconds(const negf_multi *m);
+/* Variant chosen for padded size Np: 0 matrix cores, 1 VALU, -1 not tuned yet. */
+int negf_multi_variant(int Np);
+int negf_multi_destroy(negf_multi *m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -50,7 +53,11 @@ static const double PI = 3.141592653589793;
 #define NEGF_RA 8   // rows of R accumulated per pass of the row product
 
 struct TransArgs {
-    SysView sv;
+    SysView sv;           // the device of every job, unless svs is set
+    const SysView *svs;   // uneven batches: per device, indexed by jdev[job] (same Np for all)
+    const int *jdev;
+    const int *sched_off, *sched_job;   // uneven batches: jobs of workgroup b are
+                                        // sched_job[sched_off[b] .. sched_off[b + 1])
     int njobs;
     const cd *z;          // per job E + i eta
     const int *kidx;      // per job k-point index
@@ -124,16 +131,22 @@ __device__ __forceinline__ int gje_any(cd *Z, int Np, GjeSh<PB, TWO> &s, cd *scr
     else return gje_invert2<PB>(Z, Np, s, scratch);
 }
 
-template <int MAXT, int PB, bool TWO>
+// MULTI: uneven batch -- jobs from a.sched_job, each with its device's SysView (a.svs);
+// otherwise every job uses a.sv and workgroup b runs jobs b, b + gridDim.x, ...
+template <int MAXT, int PB, bool TWO, bool MULTI = false>
 __global__ __launch_bounds__(MAXT) void k_transmission(TransArgs a) {
     __shared__ GjeSh<PB, TWO> s;
     __shared__ RowProdShared rs;
-    const SysView &sv = a.sv;
-    const int Np = sv.Np, ldR = a.nLmax;
+    const int Np = a.sv.Np, ldR = a.nLmax;
     const size_t NN = (size_t)Np * Np;
     cd *base = a.ws + (size_t)blockIdx.x * (2 * NN + 2 * (size_t)ldR * Np + 2 * (size_t)PB * Np);
     cd *scratch = base + 2 * NN + 2 * (size_t)ldR * Np;
-    for (int job = blockIdx.x; job < a.njobs; job += gridDim.x) {
+    const int i0 = MULTI ? a.sched_off[blockIdx.x] : blockIdx.x;
+    const int i1 = MULTI ? a.sched_off[blockIdx.x + 1] : a.njobs;
+    const int di = MULTI ? 1 : (int)gridDim.x;
+    for (int i = i0; i < i1; i += di) {
+        const int job = MULTI ? a.sched_job[i] : i;
+        const SysView &sv = MULTI ? a.svs[a.jdev[job]] : a.sv;
         const int nL = a.nL[job], nR = a.nR[job], off = a.open_off[job];
         if (nL == 0 || nR == 0) {             // no open channel: tau = 0 exactly
             if (threadIdx.x == 0) {
@@ -1296,7 +1309,7 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
     }
     if (Lrow.empty()) { Lrow.push_back(0); Rcol.push_back(0); gLv.push_back(0); gRv.push_back(0); }
     Temps tmp;
-    TransArgs a;
+    TransArgs a{};
     a.sv = sysview(c);
     a.njobs = nj;
     cd *d_z, *d_sl, *d_sr;
@@ -2000,6 +2013,479 @@ int negf_vhat_from_real(int N_x, int N_y, int N_z, const double *V_real, int NGX
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(vrec, dO, nv * sizeof(cd), hipMemcpyDeviceToHost));
     return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ uneven batches
+// Several devices (contexts of different N and N_z) in one submission.  Devices with the
+// same padded block size Np share one k_transmission launch: a job reads its device's
+// SysView (basis, V-hat, N_z, 2 Delta^2) through jdev, and the host hands jobs to
+// workgroups longest-first (cost N_z) so that uneven N_z finish together.  The launches
+// of the different Np run on forked streams captured into one hipGraph; negf_multi_replay
+// launches that graph again (after negf_ctx_set_potential, for instance).
+namespace {
+
+enum Variant { VAR_MFMA = 0, VAR_VALU = 1 };
+bool variant_ok(int v, int Np) { return v == VAR_VALU ? Np <= NEGF_MAXNP : Np <= 512; }
+int variant_threads(int v, int Np) { return v == VAR_VALU ? std::min(1024, round_up(Np, 64)) : round_up(2 * Np, 64); }
+int variant_pb(int v) { return v == VAR_VALU ? 16 : 32; }
+const void *variant_kernel(int v, int Np) {
+    if (v == VAR_VALU) return (const void *)k_transmission<1024, 16, false, true>;
+    return Np <= 320 ? (const void *)k_transmission<640, 32, true, true>
+                     : (const void *)k_transmission<1024, 32, true, true>;
+}
+void launch_variant(int v, int Np, int slots, hipStream_t st, const TransArgs &a) {
+    const int th = variant_threads(v, Np);
+    if (v == VAR_VALU) hipLaunchKernelGGL((k_transmission<1024, 16, false, true>), dim3(slots), dim3(th), 0, st, a);
+    else if (Np <= 320) hipLaunchKernelGGL((k_transmission<640, 32, true, true>), dim3(slots), dim3(th), 0, st, a);
+    else hipLaunchKernelGGL((k_transmission<1024, 32, true, true>), dim3(slots), dim3(th), 0, st, a);
+}
+
+// Variant picked per Np by the probe of tune_group, shared by every negf_multi.
+std::mutex g_tune_mu;
+std::map<int, int> g_tuned;
+std::map<int, std::string> g_tune_log;
+
+}  // namespace
+
+struct negf_multi {
+    std::vector<negf_ctx *> ctxs;
+    std::vector<int> npv;                  // distinct Np, one stream each
+    int device = 0, n_cu = 0;
+    hipStream_t s0 = nullptr;
+    std::vector<hipStream_t> gs;
+    std::vector<hipEvent_t> gev;
+    hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void *> bufs;              // device tables of the last call
+    struct Group {
+        int Np = 0, variant = 0, slots = 0, njobs = 0;
+        double *d_tau = nullptr;
+        int *d_st = nullptr;
+        std::vector<int> dev, je, jk;      // per job: device, energy and k index
+    };
+    std::vector<Group> groups;
+    std::vector<std::vector<double>> kw;   // per device k weights of the last call
+    double last_s = 0.0;
+};
+
+namespace {
+
+template <class T>
+int mbuf(negf_multi *m, T **p, size_t count) {
+    int st = dmalloc(p, count);
+    if (st == 0) m->bufs.push_back((void *)*p);
+    return st;
+}
+template <class T>
+int mupload(negf_multi *m, T **p, const std::vector<T> &h) {
+    int st = mbuf(m, p, h.size());
+    if (st) return st;
+    if (!h.empty()) HIPCHK(hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, m->s0));
+    return 0;
+}
+void multi_release(negf_multi *m) {
+    if (m->s0) (void)hipStreamSynchronize(m->s0);
+    if (m->exec) (void)hipGraphExecDestroy(m->exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    m->exec = nullptr;
+    m->graph = nullptr;
+    for (void *p : m->bufs) (void)hipFree(p);
+    m->bufs.clear();
+    m->groups.clear();
+}
+
+// Probe each valid variant on the group's first jobs with every device cut to its first
+// 8 slices (same Np, same per-slice work) and keep the fastest.  NEGF_MULTI_VARIANT
+// (mfma | valu) forces a choice.
+int tune_group(negf_multi *m, negf_multi::Group &g, const TransArgs &a, const std::vector<SysView> &svs,
+               int nLmax) {
+    if (const char *f = std::getenv("NEGF_MULTI_VARIANT")) {
+        const int v = std::string(f) == "valu" ? VAR_VALU : VAR_MFMA;
+        if (!variant_ok(v, g.Np)) return fail(-2, "NEGF_MULTI_VARIANT names a variant that cannot run this Np");
+        g.variant = v;
+        return 0;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(g.Np);
+        if (it != g_tuned.end()) {
+            g.variant = it->second;
+            return 0;
+        }
+    }
+    std::vector<int> cand;
+    for (int v : {VAR_MFMA, VAR_VALU})
+        if (variant_ok(v, g.Np)) cand.push_back(v);
+    int best = cand[0];
+    std::string log = "Np=" + std::to_string(g.Np) + ":";
+    if (cand.size() > 1) {
+        std::vector<SysView> cut = svs;
+        for (auto &sv : cut) sv.Nz = std::min(sv.Nz, 8);
+        SysView *d_cut;
+        int st;
+        std::vector<void *> keep;
+        if ((st = dmalloc(&d_cut, cut.size()))) return st;
+        keep.push_back(d_cut);
+        auto free_all = [&]() {
+            (void)hipStreamSynchronize(m->s0);
+            for (void *p : keep) (void)hipFree(p);
+        };
+        HIPCHK(hipMemcpyAsync(d_cut, cut.data(), cut.size() * sizeof(SysView), hipMemcpyHostToDevice, m->s0));
+        const int probe_jobs = std::min(g.njobs, m->n_cu);
+        std::vector<int> poff(probe_jobs + 1), pjob(probe_jobs);
+        for (int j = 0; j <= probe_jobs; ++j) poff[j] = j;
+        for (int j = 0; j < probe_jobs; ++j) pjob[j] = j;
+        int *d_poff, *d_pjob;
+        if ((st = dmalloc(&d_poff, poff.size())) || (keep.push_back(d_poff), false) ||
+            (st = dmalloc(&d_pjob, pjob.size())) || (keep.push_back(d_pjob), false)) {
+            free_all();
+            return st;
+        }
+        HIPCHK(hipMemcpyAsync(d_poff, poff.data(), poff.size() * sizeof(int), hipMemcpyHostToDevice, m->s0));
+        HIPCHK(hipMemcpyAsync(d_pjob, pjob.data(), pjob.size() * sizeof(int), hipMemcpyHostToDevice, m->s0));
+        double best_ms = 1e300;
+        for (int v : cand) {
+            const size_t per_slot = ((size_t)2 * g.Np * g.Np + (size_t)2 * nLmax * g.Np +
+                                     (size_t)2 * variant_pb(v) * g.Np) * sizeof(cd);
+            cd *ws;
+            double *tau;
+            int *sts;
+            if ((st = dmalloc(&ws, per_slot / sizeof(cd) * probe_jobs)) || (keep.push_back(ws), false) ||
+                (st = dmalloc(&tau, probe_jobs)) || (keep.push_back(tau), false) || (st = dmalloc(&sts, probe_jobs)) ||
+                (keep.push_back(sts), false)) {
+                free_all();
+                return st;
+            }
+            TransArgs p = a;
+            p.svs = d_cut;
+            p.sched_off = d_poff;
+            p.sched_job = d_pjob;
+            p.njobs = probe_jobs;
+            p.ws = ws;
+            p.tau = tau;
+            p.status = sts;
+            float ms = 0.f;
+            for (int rep = 0; rep < 2; ++rep) {       // first run warms up, second is timed
+                if (hipEventRecord(m->ev0, m->s0) != hipSuccess) break;
+                launch_variant(v, g.Np, probe_jobs, m->s0, p);
+                if (hipGetLastError() != hipSuccess || hipEventRecord(m->ev1, m->s0) != hipSuccess ||
+                    hipEventSynchronize(m->ev1) != hipSuccess || hipEventElapsedTime(&ms, m->ev0, m->ev1) != hipSuccess) {
+                    free_all();
+                    return fail(-11, "autotune probe launch failed");
+                }
+            }
+            log += std::string(v == VAR_MFMA ? " mfma " : " valu ") + std::to_string(ms) + " ms";
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = v;
+            }
+        }
+        free_all();
+    }
+    log += std::string(" -> ") + (best == VAR_MFMA ? "mfma" : "valu");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[g.Np] = best;
+    g_tune_log[g.Np] = log;
+    g.variant = best;
+    return 0;
+}
+
+int multi_collect(negf_multi *m, double *const *tau) {
+    HIPCHK(hipEventSynchronize(m->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    m->last_s = ms * 1e-3;
+    int status = 0;
+    for (auto &g : m->groups) {
+        std::vector<double> h_tau(g.njobs);
+        std::vector<int> h_st(g.njobs);
+        HIPCHK(hipMemcpy(h_tau.data(), g.d_tau, g.njobs * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(h_st.data(), g.d_st, g.njobs * sizeof(int), hipMemcpyDeviceToHost));
+        for (int j = 0; j < g.njobs; ++j) {
+            const int d = g.dev[j];
+            tau[d][g.je[j]] += h_tau[j] * m->kw[d][g.jk[j]];
+            if (h_st[j] > 0 && !status) status = h_st[j];
+        }
+    }
+    return status;
+}
+
+}  // namespace
+
+extern "C" {
+
+int negf_ctx_set_potential(negf_ctx *c, const double *vrec) {
+    int st = check_ctx(c);
+    if (st) return st;
+    if (!vrec) return fail(-2, "vrec is null");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t nv = (size_t)(2 * c->NGX + 1) * (2 * c->NGY + 1) * c->Nz;
+    HIPCHK(hipMemcpy(c->d_vrec, vrec, nv * sizeof(cd), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int negf_multi_create(negf_multi **out, int ndev, negf_ctx *const *ctxs) {
+    if (!out) return fail(-1, "out is null");
+    *out = nullptr;
+    if (ndev < 1 || !ctxs) return fail(-2, "need at least one context");
+    for (int d = 0; d < ndev; ++d) {
+        if (!ctxs[d]) return fail(-1, "null context in the list");
+        if (ctxs[d]->device != ctxs[0]->device) return fail(-2, "contexts of one negf_multi must share a GPU");
+        if ((ctxs[d]->flags & 3) != NEGF_PREC_FP64) return fail(-2, "uneven batches run the FP64 path only");
+    }
+    auto *m = new negf_multi();
+    m->ctxs.assign(ctxs, ctxs + ndev);
+    m->device = ctxs[0]->device;
+    m->n_cu = ctxs[0]->n_cu;
+    for (auto *c : m->ctxs)
+        if (std::find(m->npv.begin(), m->npv.end(), c->Np) == m->npv.end()) m->npv.push_back(c->Np);
+    std::sort(m->npv.begin(), m->npv.end());
+    auto bail = [&](int st) {
+        negf_multi_destroy(m);
+        return st;
+    };
+    if (hipSetDevice(m->device) != hipSuccess) return bail(fail(-8, "hipSetDevice failed"));
+    if (hipStreamCreate(&m->s0) != hipSuccess) return bail(fail(-9, "hipStreamCreate failed"));
+    for (size_t g = 0; g < m->npv.size(); ++g) {
+        hipStream_t s;
+        hipEvent_t e;
+        if (hipStreamCreate(&s) != hipSuccess) return bail(fail(-9, "hipStreamCreate failed"));
+        m->gs.push_back(s);
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return bail(fail(-9, "hipEventCreate failed"));
+        m->gev.push_back(e);
+    }
+    if (hipEventCreateWithFlags(&m->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess)
+        return bail(fail(-9, "hipEventCreate failed"));
+    *out = m;
+    return 0;
+}
+
+int negf_multi_destroy(negf_multi *m) {
+    if (!m) return 0;
+    multi_release(m);
+    for (auto e : m->gev) (void)hipEventDestroy(e);
+    for (auto s : m->gs) (void)hipStreamDestroy(s);
+    if (m->fork) (void)hipEventDestroy(m->fork);
+    if (m->ev0) (void)hipEventDestroy(m->ev0);
+    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    if (m->s0) (void)hipStreamDestroy(m->s0);
+    delete m;
+    return 0;
+}
+
+int negf_multi_transmission(negf_multi *m, const double *const *E, const int *nE, const double *const *kx,
+                            const double *const *ky, const double *const *kw, const int *nk,
+                            double *const *tau) {
+    if (!m) return fail(-1, "null negf_multi");
+    const int ndev = (int)m->ctxs.size();
+    if (!E || !nE || !kx || !ky || !kw || !nk || !tau) return fail(-2, "null argument to negf_multi_transmission");
+    for (int d = 0; d < ndev; ++d)
+        if (nE[d] < 0 || nk[d] < 1 || (nE[d] > 0 && (!E[d] || !tau[d])) || !kx[d] || !ky[d] || !kw[d])
+            return fail(-2, "bad per-device arguments to negf_multi_transmission");
+    HIPCHK(hipSetDevice(m->device));
+    multi_release(m);
+    int st;
+    m->kw.assign(ndev, {});
+    // per device job tables, as negf_transmission_batch builds them
+    std::vector<JobTables> tabs(ndev);
+    std::vector<std::vector<int>> dje(ndev), djk(ndev);
+    for (int d = 0; d < ndev; ++d) {
+        negf_ctx *c = m->ctxs[d];
+        m->kw[d].assign(kw[d], kw[d] + nk[d]);
+        const int nj = nE[d] * nk[d];
+        std::vector<std::complex<double>> energies(nE[d]);
+        std::vector<double> realE(E[d], E[d] + nE[d]);
+        for (int i = 0; i < nE[d]; ++i) energies[i] = E[d][i];
+        dje[d].resize(nj);
+        djk[d].resize(nj);
+        for (int j = 0; j < nj; ++j) {
+            dje[d][j] = j % nE[d];
+            djk[d][j] = j / nE[d];
+        }
+        if ((st = job_tables(c, energies, realE, kx[d], ky[d], nk[d], dje[d], djk[d], true, tabs[d]))) return st;
+    }
+    // one group per Np; work estimate N_z Np^3 per job gives each group its share of CUs
+    const int ng = (int)m->npv.size();
+    std::vector<double> work(ng, 0.0);
+    for (int d = 0; d < ndev; ++d) {
+        const int g = (int)(std::find(m->npv.begin(), m->npv.end(), m->ctxs[d]->Np) - m->npv.begin());
+        work[g] += (double)nE[d] * nk[d] * m->ctxs[d]->Nz * std::pow((double)m->ctxs[d]->Np, 3);
+    }
+    double wsum = 0.0;
+    for (double w : work) wsum += w;
+    std::vector<TransArgs> args(ng);
+    for (int gi = 0; gi < ng; ++gi) {
+        negf_multi::Group g;
+        g.Np = m->npv[gi];
+        const int Np = g.Np;
+        std::vector<SysView> svs;
+        std::vector<cd> z, sl, sr;
+        std::vector<double> kinh, gLv, gRv;
+        std::vector<int> kidx, jdev, off, nL, nR, Lrow, Rcol;
+        std::vector<double> cost;
+        int nLmax = 1;
+        for (int d = 0; d < ndev; ++d) {
+            negf_ctx *c = m->ctxs[d];
+            if (c->Np != Np) continue;
+            const JobTables &t = tabs[d];
+            const int dv = (int)svs.size(), krow = (int)(kinh.size() / Np);
+            svs.push_back(sysview(c));
+            kinh.insert(kinh.end(), t.kinh.begin(), t.kinh.end());
+            for (size_t j = 0; j < t.z.size(); ++j) {
+                g.dev.push_back(d);
+                g.je.push_back(dje[d][j]);
+                g.jk.push_back(djk[d][j]);
+                jdev.push_back(dv);
+                z.push_back(t.z[j]);
+                kidx.push_back(krow + t.kidx[j]);
+                sl.insert(sl.end(), t.sigL.begin() + j * Np, t.sigL.begin() + (j + 1) * Np);
+                sr.insert(sr.end(), t.sigR.begin() + j * Np, t.sigR.begin() + (j + 1) * Np);
+                off.push_back((int)Lrow.size());
+                std::vector<int> rc;
+                std::vector<double> rg;
+                int l = 0, r = 0;
+                for (int i = 0; i < c->N; ++i) {
+                    const double gl = t.gamL[j * Np + i], gr = t.gamR[j * Np + i];
+                    if (gl != 0.0) { Lrow.push_back(i); gLv.push_back(gl); ++l; }
+                    if (gr != 0.0) { rc.push_back(i); rg.push_back(gr); ++r; }
+                }
+                const int n = std::max(l, r);
+                for (int q = l; q < n; ++q) { Lrow.push_back(0); gLv.push_back(0.0); }
+                for (int q = 0; q < n; ++q) {
+                    Rcol.push_back(q < r ? rc[q] : 0);
+                    gRv.push_back(q < r ? rg[q] : 0.0);
+                }
+                nL.push_back(l);
+                nR.push_back(r);
+                nLmax = std::max(nLmax, l);
+                cost.push_back((l == 0 || r == 0) ? 0.0 : (double)c->Nz);
+            }
+        }
+        if (Lrow.empty()) { Lrow.push_back(0); Rcol.push_back(0); gLv.push_back(0); gRv.push_back(0); }
+        g.njobs = (int)z.size();
+        TransArgs &a = args[gi];
+        a = TransArgs{};
+        a.sv = svs[0];
+        a.njobs = g.njobs;
+        a.nLmax = nLmax;
+        if (g.njobs == 0) {
+            m->groups.push_back(g);
+            continue;
+        }
+        SysView *d_svs;
+        cd *d_z, *d_sl, *d_sr;
+        int *d_k, *d_jd, *d_off, *d_nL, *d_nR, *d_Lr, *d_Rc;
+        double *d_kin, *d_gl, *d_gr;
+        if ((st = mupload(m, &d_svs, svs)) || (st = mupload(m, &d_z, z)) || (st = mupload(m, &d_k, kidx)) ||
+            (st = mupload(m, &d_jd, jdev)) || (st = mupload(m, &d_kin, kinh)) || (st = mupload(m, &d_sl, sl)) ||
+            (st = mupload(m, &d_sr, sr)) || (st = mupload(m, &d_off, off)) || (st = mupload(m, &d_nL, nL)) ||
+            (st = mupload(m, &d_nR, nR)) || (st = mupload(m, &d_Lr, Lrow)) || (st = mupload(m, &d_Rc, Rcol)) ||
+            (st = mupload(m, &d_gl, gLv)) || (st = mupload(m, &d_gr, gRv)) || (st = mbuf(m, &g.d_tau, g.njobs)) ||
+            (st = mbuf(m, &g.d_st, g.njobs)))
+            return st;
+        a.svs = d_svs;
+        a.jdev = d_jd;
+        a.z = d_z;
+        a.kidx = d_k;
+        a.kinh = d_kin;
+        a.sigL = d_sl;
+        a.sigR = d_sr;
+        a.open_off = d_off;
+        a.nL = d_nL;
+        a.nR = d_nR;
+        a.Lrow = d_Lr;
+        a.Rcol = d_Rc;
+        a.gL = d_gl;
+        a.gR = d_gr;
+        a.tau = g.d_tau;
+        a.status = g.d_st;
+        if ((st = tune_group(m, g, a, svs, nLmax))) return st;
+        // workgroups: this group's share of the CUs, at most one per job
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, variant_kernel(g.variant, Np),
+                                                         variant_threads(g.variant, Np), 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        const double share = wsum > 0 ? work[gi] / wsum : 1.0;
+        g.slots = std::max(1, std::min(g.njobs, (int)std::lround(share * m->n_cu * per_cu)));
+        // longest job first onto the least-loaded workgroup
+        std::vector<int> order(g.njobs);
+        for (int j = 0; j < g.njobs; ++j) order[j] = j;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+        typedef std::pair<double, int> Load;
+        std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+        for (int b = 0; b < g.slots; ++b) heap.push(Load(0.0, b));
+        std::vector<std::vector<int>> lists(g.slots);
+        for (int j : order) {
+            Load l = heap.top();
+            heap.pop();
+            lists[l.second].push_back(j);
+            heap.push(Load(l.first + cost[j] + 1e-9, l.second));
+        }
+        std::vector<int> soff(1, 0), sjob;
+        for (auto &l : lists) {
+            sjob.insert(sjob.end(), l.begin(), l.end());
+            soff.push_back((int)sjob.size());
+        }
+        const size_t per_slot = (size_t)2 * Np * Np + (size_t)2 * nLmax * Np + (size_t)2 * variant_pb(g.variant) * Np;
+        int *d_so, *d_sj;
+        cd *d_ws;
+        if ((st = mupload(m, &d_so, soff)) || (st = mupload(m, &d_sj, sjob)) ||
+            (st = mbuf(m, &d_ws, per_slot * g.slots)))
+            return st;
+        a.sched_off = d_so;
+        a.sched_job = d_sj;
+        a.ws = d_ws;
+        m->groups.push_back(g);
+    }
+    HIPCHK(hipStreamSynchronize(m->s0));
+    // one graph: fork from s0, one launch per Np on its own stream, join into s0
+    HIPCHK(hipStreamBeginCapture(m->s0, hipStreamCaptureModeThreadLocal));
+    bool cap_ok = hipEventRecord(m->fork, m->s0) == hipSuccess;
+    for (int gi = 0; gi < ng && cap_ok; ++gi) {
+        const auto &g = m->groups[gi];
+        if (g.njobs == 0) continue;
+        cap_ok = hipStreamWaitEvent(m->gs[gi], m->fork, 0) == hipSuccess;
+        if (!cap_ok) break;
+        launch_variant(g.variant, g.Np, g.slots, m->gs[gi], args[gi]);
+        cap_ok = hipGetLastError() == hipSuccess && hipEventRecord(m->gev[gi], m->gs[gi]) == hipSuccess &&
+                 hipStreamWaitEvent(m->s0, m->gev[gi], 0) == hipSuccess;
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(m->s0, &graph);
+    if (!cap_ok || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(-12, "hipGraph capture of the uneven batch failed");
+    }
+    m->graph = graph;
+    HIPCHK(hipGraphInstantiate(&m->exec, m->graph, nullptr, nullptr, 0));
+    HIPCHK(hipEventRecord(m->ev0, m->s0));
+    HIPCHK(hipGraphLaunch(m->exec, m->s0));
+    HIPCHK(hipEventRecord(m->ev1, m->s0));
+    return multi_collect(m, tau);
+}
+
+int negf_multi_replay(negf_multi *m, double *const *tau) {
+    if (!m) return fail(-1, "null negf_multi");
+    if (!m->exec) return fail(-2, "negf_multi_replay needs a previous negf_multi_transmission");
+    if (!tau) return fail(-2, "tau is null");
+    HIPCHK(hipSetDevice(m->device));
+    HIPCHK(hipEventRecord(m->ev0, m->s0));
+    HIPCHK(hipGraphLaunch(m->exec, m->s0));
+    HIPCHK(hipEventRecord(m->ev1, m->s0));
+    return multi_collect(m, tau);
+}
+
+double negf_multi_last_seconds(const negf_multi *m) { return m ? m->last_s : -1.0; }
+
+int negf_multi_variant(int Np) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(Np);
+    return it == g_tuned.end() ? -1 : it->second;
 }
 
 }  // extern "C"

@@ -346,6 +346,14 @@ class NegfDevice:
         """Energies of the last mixed-precision transmission call recomputed in FP64."""
         return self._lib.negf_last_fallbacks(self._h)
 
+    def set_potential(self, vrec):
+        """Replace V_reciprocal_all (same window and N_z), e.g. in a self-consistent loop."""
+        v = np.asfortranarray(np.asarray(vrec, dtype=np.complex128))
+        if v.shape != self.vrec.shape:
+            raise ValueError(f"V_reciprocal_all shape {v.shape} != {self.vrec.shape}")
+        self.vrec = v
+        check(self._lib.negf_ctx_set_potential(self._h, dptr(self.vrec)), "negf_ctx_set_potential")
+
     @staticmethod
     def _kp(kpoints):
         if kpoints is None:
@@ -449,3 +457,68 @@ def ldos_reduce(LM, nx, ny, N_x, N_y, LX_ang, LY_ang, LZ_ang, LDOS_GRID):
             shape[ax] = n
             out = (out * sel.reshape(shape)).sum(axis=ax, keepdims=True)
     return out
+
+
+class Multi:
+    """Uneven batch (extension): several NegfDevice systems of one GPU, any N and N_z,
+    computed in one hipGraph by negf_multi_transmission.  Each device's result is that of
+    its own NegfDevice.transmission."""
+
+    def __init__(self, devices):
+        self.devices = list(devices)
+        self._lib = _lib.load()
+        hs = (ctypes.c_void_p * len(self.devices))(*[d._h.value for d in self.devices])
+        h = ctypes.c_void_p()
+        check(self._lib.negf_multi_create(ctypes.byref(h), len(self.devices), hs), "negf_multi_create")
+        self._h = h
+        self._tau = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.negf_multi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ptrs(self, arrays):
+        return (ctypes.POINTER(ctypes.c_double) * len(arrays))(*[dptr(a) for a in arrays])
+
+    def transmission(self, energies, kpoints=None):
+        """energies: one array per device (Hartree); kpoints: None (Gamma only) or one
+        (kx, ky, kw) per device.  Returns one T(E) array per device."""
+        n = len(self.devices)
+        if len(energies) != n:
+            raise ValueError("one energy array per device")
+        self._E = [np.ascontiguousarray(e, dtype=np.float64) for e in energies]
+        kps = [NegfDevice._kp(None if kpoints is None else kpoints[d]) for d in range(n)]
+        self._kx, self._ky, self._kw = ([k[i] for k in kps] for i in range(3))
+        nE = np.array([len(e) for e in self._E], dtype=np.int32)
+        nk = np.array([len(k) for k in self._kx], dtype=np.int32)
+        self._tau = [np.zeros(len(e)) for e in self._E]
+        self.last_status = check(self._lib.negf_multi_transmission(
+            self._h, self._ptrs(self._E), iptr(nE), self._ptrs(self._kx), self._ptrs(self._ky),
+            self._ptrs(self._kw), iptr(nk), self._ptrs(self._tau)), "negf_multi_transmission")
+        return [t.copy() for t in self._tau]
+
+    def replay(self):
+        """Launch the last call's hipGraph again (after set_potential on any device)."""
+        if self._tau is None:
+            raise RuntimeError("replay needs a previous transmission call")
+        tau = [np.zeros_like(t) for t in self._tau]
+        self.last_status = check(self._lib.negf_multi_replay(self._h, self._ptrs(tau)), "negf_multi_replay")
+        return tau
+
+    @property
+    def last_seconds(self):
+        return self._lib.negf_multi_last_seconds(self._h)
+
+    @staticmethod
+    def variant(Np):
+        """Inversion variant chosen for padded size Np: 'mfma', 'valu' or None."""
+        v = _lib.load().negf_multi_variant(int(Np))
+        return {0: "mfma", 1: "valu"}.get(v)
+

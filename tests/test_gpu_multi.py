@@ -1,0 +1,164 @@
+"""Uneven batches (negf_multi, extension) through the C-ABI.
+
+Several devices with different N and N_z in one hipGraph.  Bars, each written in its
+assertion:
+- with the matrix-core variant forced, every device's T equals its own single-device
+  negf_transmission_batch result bit for bit (same kernel, same arithmetic);
+- with the VALU variant forced, and with the autotuned choice, every device matches the
+  CPU oracle to relative 1e-10;
+- graph replays after a potential update equal eager single-device runs with the new
+  potential, on every replay.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import negf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def ng():
+    from negftrans_amd import negf
+    if negf.device_count() < 1:
+        pytest.fail("no GPU visible to libnegf_gpu.so")
+    return negf
+
+
+@pytest.fixture
+def variant():
+    """Set NEGF_MULTI_VARIANT for one test and restore it afterwards."""
+    old = os.environ.get("NEGF_MULTI_VARIANT")
+
+    def set_(v):
+        if v is None:
+            os.environ.pop("NEGF_MULTI_VARIANT", None)
+        else:
+            os.environ["NEGF_MULTI_VARIANT"] = v
+    yield set_
+    set_(old)
+
+
+def system(R2, N_z, seed, L=12.0, NK=(1, 1), eta=1e-6, amp=0.02):
+    ENCUT = R2 * (2 * np.pi / L) ** 2 / 2
+    nx, ny = O.planewave_basis(ENCUT, L, L)
+    NGX = NGY = 2 * int(np.ceil(np.sqrt(R2))) + 1
+    N_x = N_y = 2 * NGX + 4
+    V = amp * np.random.default_rng(seed).standard_normal((N_x, N_y, N_z)) + 0.01
+    Vrec = O.v_reciprocal_all(V, NGX, NGY)
+    a = O.Atomic(ETA=eta, LX=L, LY=L, LZ=N_z * 0.7, ENCUT=ENCUT)
+    kx, ky, kw = O.kpoint_mesh(NK[0], NK[1], L, L)
+    return O.System(a, nx, ny, Vrec, NGX, NGY, kx, ky, kw)
+
+
+def device(ng, s):
+    a = s.atomic
+    at = ng.Atomic(V_L=a.V_L, V_R=a.V_R, ETA=a.ETA, LX=a.LX, LY=a.LY, LZ=a.LZ, ENCUT=a.ENCUT)
+    return ng.NegfDevice(s.nx, s.ny, at, s.Vrec, s.NGX, s.NGY)
+
+
+# four devices, three padded sizes (32, 32, 64, 128), N_z from 3 to 14
+SMALL = [dict(R2=6.5, N_z=9, seed=1), dict(R2=4.5, N_z=14, seed=2, NK=(2, 1)),
+         dict(R2=12.5, N_z=3, seed=3), dict(R2=30.5, N_z=6, seed=4)]
+
+
+def small_set(ng):
+    systems = [system(**p) for p in SMALL]
+    devs = [device(ng, s) for s in systems]
+    energies = [np.linspace(0.05, 1.1, 7 + 2 * i) for i in range(len(systems))]
+    kps = [(s.kx, s.ky, s.kw) for s in systems]
+    return systems, devs, energies, kps
+
+
+def test_multi_mfma_equals_single_device_runs(ng, variant):
+    variant("mfma")
+    systems, devs, energies, kps = small_set(ng)
+    assert len({d.N for d in devs}) == 4
+    got = ng.Multi(devs).transmission(energies, kps)
+    for d, E, kp, g in zip(devs, energies, kps, got):
+        ref = d.transmission(E, kp)
+        assert np.array_equal(g, ref), (d.N, np.max(np.abs(g - ref)))
+
+
+@pytest.mark.parametrize("v", ["valu", None])
+def test_multi_vs_oracle(ng, variant, v):
+    variant(v)
+    systems, devs, energies, kps = small_set(ng)
+    m = ng.Multi(devs)
+    got = m.transmission(energies, kps)
+    assert m.last_status == 0 and m.last_seconds > 0
+    for s, E, g in zip(systems, energies, got):
+        ref = O.transmission(s, E)
+        np.testing.assert_allclose(g, ref, rtol=RTOL, atol=1e-12 * max(1.0, np.abs(ref).max()))
+    if v is None:
+        for d in devs:
+            assert ng.Multi.variant(-(-d.N // 32) * 32) in ("mfma", "valu")
+
+
+def test_multi_replay_after_potential_update(ng, variant):
+    variant("mfma")
+    systems, devs, energies, kps = small_set(ng)
+    m = ng.Multi(devs)
+    first = m.transmission(energies, kps)
+    rng = np.random.default_rng(99)
+    for rep in range(3):
+        # eager work on the GPU between replays: a new potential for device 1 and a
+        # single-device run of device 0
+        s = systems[1]
+        N_x = 2 * s.NGX + 4
+        V = 0.02 * rng.standard_normal((N_x, N_x, s.N_z)) + 0.01
+        devs[1].set_potential(O.v_reciprocal_all(V, s.NGX, s.NGY))
+        eager0 = devs[0].transmission(energies[0], kps[0])
+        got = m.replay()
+        assert m.last_status == 0
+        assert np.array_equal(got[0], first[0]) and np.array_equal(got[0], eager0)
+        assert np.array_equal(got[1], devs[1].transmission(energies[1], kps[1]))
+        assert not np.array_equal(got[1], first[1])
+        for i in (2, 3):
+            assert np.array_equal(got[i], first[i])
+
+
+def test_multi_large_block_valu_path(ng):
+    """N > 512 (padded 16-wide, one thread per row, the only variant there) next to a
+    small device, against the oracle."""
+    big = system(R2=170.5, N_z=3, seed=6, eta=1e-5)
+    assert big.N > 512
+    small = system(R2=6.5, N_z=5, seed=7)
+    devs = [device(ng, big), device(ng, small)]
+    E = [np.array([0.3, 0.9]), np.array([0.2, 0.6, 1.0])]
+    got = ng.Multi(devs).transmission(E)
+    for s, e, g in zip((big, small), E, got):
+        ref = O.transmission(s, e)
+        np.testing.assert_allclose(g, ref, rtol=RTOL, atol=1e-12 * max(1.0, np.abs(ref).max()))
+    single = devs[0].transmission(E[0])
+    np.testing.assert_allclose(single, O.transmission(big, E[0]), rtol=RTOL)
+
+
+def test_multi_survey_sizes_vs_single(ng, variant):
+    """The uneven-batch block sizes N in {61, 113, 197, 317, 509} with N_z from 40 to
+    300, 12 energies each, autotuned: every device matches its single-device run to
+    relative 1e-10 (bit for bit where the matrix-core variant was chosen)."""
+    variant(None)
+    from negftrans_amd import workloads
+    params = [(61, 300), (113, 40), (197, 160), (317, 80), (509, 60)]
+    systems, devs, energies = [], [], []
+    for i, (N, N_z) in enumerate(params):
+        R2 = {61: 18.5, 113: 36.5, 197: 64.5, 317: 100.5, 509: 162.5}[N]
+        s = system(R2=R2, N_z=N_z, seed=10 + i, L=200.0, amp=2e-5, eta=1e-8)
+        assert s.N == N
+        systems.append(s)
+        devs.append(device(ng, s))
+        kin_max = O.kinetic(s.nx, s.ny, s.atomic.LX, s.atomic.LY, 0, 0).max()
+        energies.append(np.linspace(0.0005, 0.3 * kin_max, 12))
+    m = ng.Multi(devs)
+    got = m.transmission(energies)
+    assert m.last_status == 0
+    for d, E, g in zip(devs, energies, got):
+        ref = d.transmission(E)
+        np.testing.assert_allclose(g, ref, rtol=RTOL, atol=1e-13 * max(1.0, np.abs(ref).max()))
+        if ng.Multi.variant(-(-d.N // 32) * 32) == "mfma":
+            assert np.array_equal(g, ref)

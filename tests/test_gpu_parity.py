@@ -32,7 +32,7 @@ def rand_c(rng, *shape):
 
 
 # ------------------------------------------------------------------ a7, a8
-@pytest.mark.parametrize("N", [1, 5, 16, 17, 61, 100, 317])
+@pytest.mark.parametrize("N", [1, 5, 16, 17, 61, 100, 317, 600, 797])
 def test_inverse_batch(ng, N):
     rng = np.random.default_rng(N)
     A = rand_c(rng, N, N, 3)
