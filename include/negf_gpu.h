@@ -17,6 +17,8 @@
 #ifndef NEGF_GPU_H
 #define NEGF_GPU_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -214,6 +216,38 @@ double negf_multi_last_seconds(const negf_multi *m);
 /* Variant chosen for padded size Np: 0 matrix cores, 1 VALU, -1 not tuned yet. */
 int negf_multi_variant(int Np);
 int negf_multi_destroy(negf_multi *m);
+
+/*
+ * Extension, no reference counterpart: kernels of the learned adaptive energy grid (a
+ * multilayer perceptron trained on T(E) curves this library computes).  Device pointers,
+ * FP32 row-major storage with leading dimensions, work queued on `stream` (a hipStream_t;
+ * NULL = the default stream).  Products round both operands to bfloat16 (nearest even)
+ * and accumulate in FP32 on the matrix cores.
+ *
+ * negf_mlp_gemm: C (M x N) = op(A) (M x K) op(B) (K x N); op(A)(i,k) = transA ? A[k lda + i]
+ *   : A[i lda + k], likewise B.  mode 0: C = product (+ C if accumulate); mode 1:
+ *   Z = product + bias[j], C = GELU(Z) (exact erf form); mode 2: C = product + bias[j].
+ * negf_mlp_gelu_backward: dZ = dY GELU'(Z) over n entries.
+ * negf_mlp_colsum: out[j] = sum_i X[i ldx + j].
+ * negf_mlp_nll: Gaussian negative log-likelihood of T (B x P) under O (B x 2P) = [mean |
+ *   log-variance clamped to +-20]: *loss += mean over B P of 0.5 ((T - mu)^2 e^-s + s);
+ *   G (B x 2P) = d loss / d O.
+ * negf_mlp_adam: w -= lr_t m / (sqrt(v) + eps) after the moment updates (lr_t carries the
+ *   bias correction).
+ */
+int negf_mlp_gemm(int M, int N, int K, const float *A, int lda, int transA, const float *B, int ldb, int transB,
+                  float *C, int ldc, const float *bias, float *Z, int mode, int accumulate, void *stream);
+int negf_mlp_gelu_backward(int n, const float *Z, const float *dY, float *dZ, void *stream);
+int negf_mlp_colsum(int M, int N, const float *X, int ldx, float *out, void *stream);
+int negf_mlp_nll(int B, int P, const float *O, const float *T, float *G, float *loss, void *stream);
+int negf_mlp_adam(int n, float *w, const float *g, float *m, float *v, float lr_t, float b1, float b2, float eps,
+                  void *stream);
+/* Device buffers for the entry points above: allocate, free, and a synchronous copy
+ * (kind 0 host -> device, 1 device -> host, 2 device -> device; it waits for the work
+ * queued before it on the default stream). */
+int negf_dev_alloc(void **ptr, size_t bytes);
+int negf_dev_free(void *ptr);
+int negf_dev_copy(void *dst, const void *src, size_t bytes, int kind);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,17 @@ SIGNATURES = {
     "negf_multi_last_seconds": (ctypes.c_double, [_vp]),
     "negf_multi_variant": (ctypes.c_int, [ctypes.c_int]),
     "negf_multi_destroy": (ctypes.c_int, [_vp]),
+    # device-pointer entry points of the adaptive-grid network (pointers as integers)
+    "negf_mlp_gemm": (ctypes.c_int, [ctypes.c_int] * 3 + [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
+                                     ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
+    "negf_mlp_gelu_backward": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp]),
+    "negf_mlp_colsum": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]),
+    "negf_mlp_nll": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "negf_mlp_adam": (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_float, _vp]),
+    "negf_dev_alloc": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_size_t]),
+    "negf_dev_free": (ctypes.c_int, [_vp]),
+    "negf_dev_copy": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_int]),
     "negf_hamiltonian_direct": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
                                                ctypes.c_int, ctypes.c_int, _dp]),

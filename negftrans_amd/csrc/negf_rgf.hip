@@ -34,6 +34,10 @@ static int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
+// Shared with the other translation units of the library (not exported).
+__attribute__((visibility("hidden"))) int negf_fail_internal(int code, const std::string &msg) {
+    return fail(code, msg);
+}
 #define HIPCHK(x)                                                                      \
     do {                                                                               \
         hipError_t e_ = (x);                                                           \

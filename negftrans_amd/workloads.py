@@ -17,6 +17,12 @@ Configurations (all single-device, k = 0):
             64 x 64 x 1000 grid of Gaussian-correlated disorder (sigma = 3 meV,
             correlation length 5 nm, periodic in x, y, seed 20261203), tapered to zero
             over 10 nm at both z ends; V_L = V_R = 0; 4096 energies in 0-30 meV.
+  qpc_c4_instance(seed)  the learned adaptive grid's instances: qpc_c0's form with
+            N = 113 (R^2 = 36.5), N_z = 400 slices of 0.25 nm on a 32 x 32 grid, seeded
+            V0 in 6-14 meV, w in 6-10 nm, saddle centre 40-60 nm and width 8-12 nm,
+            confinement 1.5-2.5 meV, plus Gaussian-correlated disorder (sigma 0-0.5 meV,
+            5 nm); energies in 0-20 meV.  Features: the 16 x 16 x 8 lowest Fourier
+            components of V, returned as the band-limited real samples they define.
 """
 from __future__ import annotations
 
@@ -115,3 +121,36 @@ def disorder_c1(n_energy=4096, N_z=1000, R2=100.5, grid=64, sigma=0.003, seed=20
                ETA=ms * 1e-6 / HARTREE, TRANSMISSION_GRID=(0.0, ms * 0.030, n_energy))
     E = uniform_energies(0.0, 0.030, n_energy)
     return Workload("disorder_c1", inp, ms * V, E)
+
+
+def qpc_c4_instance(seed, n_energy=512, N_z=400, R2=36.5, grid=32) -> Workload:
+    rng = np.random.default_rng(seed)
+    Lx = Ly = 400.0
+    LZ = 1000.0
+    dz = LZ / N_z
+    ms = MSTAR_GAAS
+    x = (np.arange(grid) * Lx / grid)[:, None, None]
+    y = (np.arange(grid) * Ly / grid)[None, :, None]
+    z = ((np.arange(N_z) + 0.5) * dz)[None, None, :]
+    V0, w = rng.uniform(0.006, 0.014), rng.uniform(60.0, 100.0)
+    z0, zw = rng.uniform(400.0, 600.0), rng.uniform(80.0, 120.0)
+    hw = rng.uniform(0.0015, 0.0025)
+    saddle = V0 * np.exp(-(z - z0) ** 2 / (2 * zw ** 2)) * (x - Lx / 2) ** 2 / w ** 2
+    conf = 0.5 * hw ** 2 / (HBAR2_OVER_ME / ms) * (y - Ly / 2) ** 2
+    dis = gaussian_disorder(grid, N_z, Lx, dz, rng.uniform(0.0, 0.0005), 50.0, seed + 7919, 100.0)
+    V = saddle + conf + dis
+    inp = dict(if_transmission=True, LX=Lx, LY=Ly, LZ=LZ, NKX=1, NKY=1, ENCUT=encut_for(R2, Lx),
+               ETA=ms * 1e-6 / HARTREE, TRANSMISSION_GRID=(0.0, ms * 0.020, n_energy))
+    E = uniform_energies(0.0, 0.020, n_energy)
+    return Workload("qpc_c4", inp, ms * V, E, extra=dict(V0=V0, w=w, z0=z0, zw=zw, hw=hw))
+
+
+def fourier_features(V_eV, shape=(16, 16, 8), scale=0.010 * MSTAR_GAAS):
+    """The shape[0] x shape[1] x shape[2] lowest Fourier components of V (eV), as the
+    band-limited real samples on a grid of that shape, divided by `scale`."""
+    F = np.fft.fftn(V_eV)
+    idx = [np.r_[0:s // 2, n - s // 2:n] for s, n in zip(shape, V_eV.shape)]
+    sub = F[np.ix_(*idx)]
+    out = np.real(np.fft.ifftn(sub)) * (np.prod(shape) / np.prod(V_eV.shape))
+    return (out / scale).astype(np.float32).ravel()
+
