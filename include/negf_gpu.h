@@ -211,7 +211,15 @@ int negf_multi_transmission(negf_multi *m, const double *const *E, const int *nE
 /* Launch the graph of the last negf_multi_transmission again (same jobs; the contexts'
  * potentials may have been replaced) and accumulate into tau the same way. */
 int negf_multi_replay(negf_multi *m, double *const *tau);
-/* Device time of the last negf_multi_transmission or replay (HIP events), seconds. */
+/* LDOS of the uneven batch at selected slices: for every device d, energy e and kept
+ * slice q (slices[d][q], 0-based), ldos[d] (N_d x N_d x nsel x nE[d] complex128,
+ * overwritten) receives -sum_k kw[d][k] / pi G_kk(E[d][e] + i eta) at that slice -- the
+ * same blocks negf_ldos_batch gives for all slices.  One hipGraph as above (the outputs
+ * are zeroed by a kernel inside it). */
+int negf_multi_ldos(negf_multi *m, const double *const *E, const int *nE, const double *const *kx,
+                    const double *const *ky, const double *const *kw, const int *nk, int nsel,
+                    const int *const *slices, double *const *ldos);
+/* Device time of the last negf_multi_transmission, replay or LDOS call (HIP events), seconds. */
 double negf_multi_last_seconds(const negf_multi *m);
 /* Variant chosen for padded size Np: 0 matrix cores, 1 VALU, -1 not tuned yet. */
 int negf_multi_variant(int Np);

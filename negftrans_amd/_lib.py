@@ -46,6 +46,9 @@ SIGNATURES = {
                                                ctypes.POINTER(_dp), ctypes.POINTER(_dp), _ip,
                                                ctypes.POINTER(_dp)]),
     "negf_multi_replay": (ctypes.c_int, [_vp, ctypes.POINTER(_dp)]),
+    "negf_multi_ldos": (ctypes.c_int, [_vp, ctypes.POINTER(_dp), _ip, ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                                       ctypes.POINTER(_dp), _ip, ctypes.c_int, ctypes.POINTER(_ip),
+                                       ctypes.POINTER(_dp)]),
     "negf_multi_last_seconds": (ctypes.c_double, [_vp]),
     "negf_multi_variant": (ctypes.c_int, [ctypes.c_int]),
     "negf_multi_destroy": (ctypes.c_int, [_vp]),

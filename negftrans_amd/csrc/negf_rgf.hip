@@ -507,6 +507,15 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
 // and out(:, :, k, tgt) += wgt * G_kk for the N x N physical block.
 struct DiagArgs {
     SysView sv;
+    // uneven batches (MULTI): per-job device index into svs, per-workgroup job lists,
+    // per-device output bases (N x N x nsel[d] per target), slice -> output slot maps
+    // (smap[smap_off[d] + k], -1 = not kept) and the per-workgroup workspace stride
+    const SysView *svs;
+    const int *jdev;
+    const int *sched_off, *sched_job;
+    cd *const *outs;
+    const int *nsel, *smap, *smap_off;
+    size_t ws_stride;
     int njobs;
     const cd *z;
     const int *kidx;
@@ -519,23 +528,35 @@ struct DiagArgs {
     int *status;
 };
 
-template <int MAXT, int PB, bool TWO>
+template <int MAXT, int PB, bool TWO, bool MULTI = false>
 __global__ __launch_bounds__(MAXT) void k_diagonal(DiagArgs a) {
     __shared__ GjeSh<PB, TWO> s;
-    const SysView &sv = a.sv;
-    const int Np = sv.Np, N = sv.N, Nz = sv.Nz;
+    const int Np = a.sv.Np;
     const size_t NN = (size_t)Np * Np;
-    cd *G = a.ws + (size_t)blockIdx.x * ((Nz + 2) * NN + 2 * (size_t)PB * Np);   // G[m] = g_{m+1}
-    cd *scratch = G + (Nz + 2) * NN;
-    cd *X = G, *Y = G + (size_t)Nz * NN, *W = G + (size_t)(Nz + 1) * NN;
-    for (int job = blockIdx.x; job < a.njobs; job += gridDim.x) {
+    const size_t stride = MULTI ? a.ws_stride : (size_t)(a.sv.Nz + 2) * NN + 2 * (size_t)PB * Np;
+    cd *G = a.ws + (size_t)blockIdx.x * stride;   // G[m] = g_{m+1}
+    const int i0 = MULTI ? a.sched_off[blockIdx.x] : blockIdx.x;
+    const int i1 = MULTI ? a.sched_off[blockIdx.x + 1] : a.njobs;
+    const int di = MULTI ? 1 : (int)gridDim.x;
+    for (int i = i0; i < i1; i += di) {
+        const int job = MULTI ? a.sched_job[i] : i;
+        const int dv = MULTI ? a.jdev[job] : 0;
+        const SysView &sv = MULTI ? a.svs[dv] : a.sv;
+        const int N = sv.N, Nz = sv.Nz;
+        cd *scratch = G + (size_t)(Nz + 2) * NN;
+        cd *X = G, *Y = G + (size_t)Nz * NN, *W = G + (size_t)(Nz + 1) * NN;
         JobView jv;
         jv.z = a.z[job];
         jv.kinh = a.kinh + (size_t)a.kidx[job] * Np;
         jv.sigL = a.sigL + (size_t)job * Np;
         jv.sigR = a.sigR + (size_t)job * Np;
         const cd wgt = a.wgt[job];
-        cd *out = a.out + (size_t)a.target[job] * N * N * Nz;
+        const int nsel = MULTI ? a.nsel[dv] : Nz;
+        cd *out = (MULTI ? a.outs[dv] : a.out) + (size_t)a.target[job] * N * N * nsel;
+        auto slice_out = [&](int k) -> cd * {
+            const int q = MULTI ? a.smap[a.smap_off[dv] + k] : k;
+            return q < 0 ? nullptr : out + (size_t)q * N * N;
+        };
         int bad = 0;
         // backward generators g_{N_z} .. g_2 stored unpermuted; the last pass leaves
         // G_11 = (D_1 - g_2)^{-1} in X
@@ -546,27 +567,29 @@ __global__ __launch_bounds__(MAXT) void k_diagonal(DiagArgs a) {
             copy_cols(m >= 2 ? G + (size_t)(m - 1) * NN : X, W, s.sigma, Np);
             __syncthreads();
         }
-        for (int col = 0; col < N; ++col)
-            for (int r = threadIdx.x; r < N; r += blockDim.x) {
-                const cd v = cmul(wgt, X[r + (size_t)col * Np]);
-                double *o = reinterpret_cast<double *>(out + r + (size_t)col * N);
-                atomicAdd(o, v.x);
-                atomicAdd(o + 1, v.y);
-            }
+        if (cd *o0 = slice_out(0)) {
+            for (int col = 0; col < N; ++col)
+                for (int r = threadIdx.x; r < N; r += blockDim.x) {
+                    const cd v = cmul(wgt, X[r + (size_t)col * Np]);
+                    double *o = reinterpret_cast<double *>(o0 + r + (size_t)col * N);
+                    atomicAdd(o, v.x);
+                    atomicAdd(o + 1, v.y);
+                }
+        }
         __syncthreads();
         // G_kk = g_k (G_{k-1,k-1} g_k + I)  (two products per slice, one call site)
         for (int step = 2; step < 2 * Nz; ++step) {
             const int k = step >> 1;          // 0-based slice, generator g_{k+1} at G[k]
             const bool second = step & 1;
             const cd *gk = G + (size_t)k * NN;
-            cd *ok = out + (size_t)k * N * N;
+            cd *ok = slice_out(k);
             gemm_nn(second ? gk : X, second ? Y : gk, nullptr, Np, [&](int r, int c, cd v) {
                 if (!second) {
                     if (r == c) v.x += 1.0;
                     Y[r + (size_t)c * Np] = v;
                 } else {
                     X[r + (size_t)c * Np] = v;
-                    if (r < N && c < N) {
+                    if (ok && r < N && c < N) {
                         const cd t = cmul(wgt, v);
                         double *o = reinterpret_cast<double *>(ok + r + (size_t)c * N);
                         atomicAdd(o, t.x);
@@ -1476,7 +1499,7 @@ static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &e
     JobTables t;
     if ((st = job_tables(c, energies, realE, kx, ky, nk, je, jk, false, t))) return st;
     Temps tmp;
-    DiagArgs a;
+    DiagArgs a{};
     a.sv = sysview(c);
     a.njobs = nj;
     cd *d_z, *d_sl, *d_sr, *d_w, *d_out;
@@ -2073,6 +2096,10 @@ struct negf_multi {
     std::vector<Group> groups;
     std::vector<std::vector<double>> kw;   // per device k weights of the last call
     double last_s = 0.0;
+    // LDOS at selected slices (negf_multi_ldos): its own graph and buffers
+    hipGraph_t lgraph = nullptr;
+    hipGraphExec_t lexec = nullptr;
+    std::vector<void *> lbufs;
 };
 
 namespace {
@@ -2090,6 +2117,22 @@ int mupload(negf_multi *m, T **p, const std::vector<T> &h) {
     if (!h.empty()) HIPCHK(hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, m->s0));
     return 0;
 }
+void multi_release_ldos(negf_multi *m) {
+    if (m->s0) (void)hipStreamSynchronize(m->s0);
+    if (m->lexec) (void)hipGraphExecDestroy(m->lexec);
+    if (m->lgraph) (void)hipGraphDestroy(m->lgraph);
+    m->lexec = nullptr;
+    m->lgraph = nullptr;
+    for (void *p : m->lbufs) (void)hipFree(p);
+    m->lbufs.clear();
+}
+
+// Zero n complex entries (a kernel rather than a memset node, so it can sit in a graph).
+__global__ __launch_bounds__(256) void k_zero_cd(cd *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_double2(0.0, 0.0);
+}
+
 void multi_release(negf_multi *m) {
     if (m->s0) (void)hipStreamSynchronize(m->s0);
     if (m->exec) (void)hipGraphExecDestroy(m->exec);
@@ -2272,6 +2315,7 @@ int negf_multi_create(negf_multi **out, int ndev, negf_ctx *const *ctxs) {
 int negf_multi_destroy(negf_multi *m) {
     if (!m) return 0;
     multi_release(m);
+    multi_release_ldos(m);
     for (auto e : m->gev) (void)hipEventDestroy(e);
     for (auto s : m->gs) (void)hipStreamDestroy(s);
     if (m->fork) (void)hipEventDestroy(m->fork);
@@ -2471,6 +2515,240 @@ int negf_multi_transmission(negf_multi *m, const double *const *E, const int *nE
     HIPCHK(hipGraphLaunch(m->exec, m->s0));
     HIPCHK(hipEventRecord(m->ev1, m->s0));
     return multi_collect(m, tau);
+}
+
+int negf_multi_ldos(negf_multi *m, const double *const *E, const int *nE, const double *const *kx,
+                    const double *const *ky, const double *const *kw, const int *nk, int nsel,
+                    const int *const *slices, double *const *ldos) {
+    if (!m) return fail(-1, "null negf_multi");
+    const int ndev = (int)m->ctxs.size();
+    if (!E || !nE || !kx || !ky || !kw || !nk || !slices || !ldos || nsel < 1)
+        return fail(-2, "null argument to negf_multi_ldos");
+    for (int d = 0; d < ndev; ++d) {
+        if (nE[d] < 0 || nk[d] < 1 || (nE[d] > 0 && (!E[d] || !ldos[d])) || !kx[d] || !ky[d] || !kw[d] || !slices[d])
+            return fail(-2, "bad per-device arguments to negf_multi_ldos");
+        for (int q = 0; q < nsel; ++q)
+            if (slices[d][q] < 0 || slices[d][q] >= m->ctxs[d]->Nz) return fail(-2, "slice index out of range");
+    }
+    HIPCHK(hipSetDevice(m->device));
+    multi_release_ldos(m);
+    int st;
+    auto lbuf = [&](auto **p, size_t count) {
+        int r = dmalloc(p, count);
+        if (r == 0) m->lbufs.push_back((void *)*p);
+        return r;
+    };
+    auto lup = [&](auto **p, const auto &h) {
+        int r = lbuf(p, h.size());
+        if (r) return r;
+        if (!h.empty())
+            HIPCHK(hipMemcpyAsync(*p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice, m->s0));
+        return 0;
+    };
+    std::vector<JobTables> tabs(ndev);
+    std::vector<std::vector<int>> dje(ndev), djk(ndev);
+    for (int d = 0; d < ndev; ++d) {
+        negf_ctx *c = m->ctxs[d];
+        const int nj = nE[d] * nk[d];
+        std::vector<std::complex<double>> energies(nE[d]);
+        std::vector<double> realE(E[d], E[d] + nE[d]);
+        for (int i = 0; i < nE[d]; ++i) energies[i] = E[d][i];
+        for (int j = 0; j < nj; ++j) {      // jobs in negf_ldos_batch's order
+            dje[d].push_back(j % std::max(1, nE[d]));
+            djk[d].push_back(j / std::max(1, nE[d]));
+        }
+        if ((st = job_tables(c, energies, realE, kx[d], ky[d], nk[d], dje[d], djk[d], false, tabs[d]))) return st;
+    }
+    // device outputs: N x N x nsel x nE complex, zeroed inside the graph
+    std::vector<cd *> d_out(ndev, nullptr);
+    std::vector<size_t> nout(ndev, 0);
+    for (int d = 0; d < ndev; ++d) {
+        nout[d] = (size_t)m->ctxs[d]->N * m->ctxs[d]->N * nsel * std::max(0, nE[d]);
+        if ((st = lbuf(&d_out[d], std::max<size_t>(nout[d], 1)))) return st;
+    }
+    const int ng = (int)m->npv.size();
+    std::vector<double> work(ng, 0.0);
+    for (int d = 0; d < ndev; ++d) {
+        const int g = (int)(std::find(m->npv.begin(), m->npv.end(), m->ctxs[d]->Np) - m->npv.begin());
+        work[g] += (double)nE[d] * nk[d] * m->ctxs[d]->Nz * std::pow((double)m->ctxs[d]->Np, 3);
+    }
+    double wsum = 0.0;
+    for (double w : work) wsum += w;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    struct LGroup {
+        int Np, slots, njobs;
+        DiagArgs a;
+        std::vector<int> devs;
+    };
+    std::vector<LGroup> groups;
+    for (int gi = 0; gi < ng; ++gi) {
+        LGroup g;
+        g.Np = m->npv[gi];
+        const int Np = g.Np;
+        std::vector<SysView> svs;
+        std::vector<cd> z, sl, sr, wg;
+        std::vector<double> kinh, cost;
+        std::vector<int> kidx, jdev, tgt, nsv, smap, smoff;
+        int maxNz = 0;
+        for (int d = 0; d < ndev; ++d) {
+            negf_ctx *c = m->ctxs[d];
+            if (c->Np != Np) continue;
+            g.devs.push_back(d);
+            const JobTables &t = tabs[d];
+            const int dv = (int)svs.size(), krow = (int)(kinh.size() / Np);
+            svs.push_back(sysview(c));
+            maxNz = std::max(maxNz, c->Nz);
+            kinh.insert(kinh.end(), t.kinh.begin(), t.kinh.end());
+            nsv.push_back(nsel);
+            smoff.push_back((int)smap.size());
+            std::vector<int> mp(c->Nz, -1);
+            for (int q = 0; q < nsel; ++q) mp[slices[d][q]] = q;
+            smap.insert(smap.end(), mp.begin(), mp.end());
+            const double delta = c->Lz / c->Nz, sc = 2 * delta * delta;
+            for (size_t j = 0; j < t.z.size(); ++j) {
+                jdev.push_back(dv);
+                z.push_back(t.z[j]);
+                kidx.push_back(krow + t.kidx[j]);
+                sl.insert(sl.end(), t.sigL.begin() + j * Np, t.sigL.begin() + (j + 1) * Np);
+                sr.insert(sr.end(), t.sigR.begin() + j * Np, t.sigR.begin() + (j + 1) * Np);
+                wg.push_back(make_double2(-kw[d][djk[d][j]] / PI * sc, 0.0));   // src/negf.f90:230-231
+                tgt.push_back(dje[d][j]);
+                cost.push_back((double)c->Nz);
+            }
+        }
+        g.njobs = (int)z.size();
+        g.slots = 0;
+        DiagArgs &a = g.a;
+        a = DiagArgs{};
+        if (g.njobs > 0) {
+            a.sv = svs[0];
+            a.sv.Nz = maxNz;
+            a.njobs = g.njobs;
+            // per-device output pointers, indexed by the group-local device index
+            std::vector<cd *> go;
+            for (int d : g.devs) go.push_back(d_out[d]);
+            SysView *d_svs;
+            cd *d_z, *d_sl, *d_sr, *d_w;
+            cd **d_go;
+            int *d_k, *d_jd, *d_t, *d_ns, *d_sm, *d_so2, *d_st;
+            double *d_kin;
+            if ((st = lup(&d_svs, svs)) || (st = lup(&d_z, z)) || (st = lup(&d_k, kidx)) || (st = lup(&d_jd, jdev)) ||
+                (st = lup(&d_kin, kinh)) || (st = lup(&d_sl, sl)) || (st = lup(&d_sr, sr)) || (st = lup(&d_w, wg)) ||
+                (st = lup(&d_t, tgt)) || (st = lup(&d_go, go)) || (st = lup(&d_ns, nsv)) || (st = lup(&d_sm, smap)) ||
+                (st = lup(&d_so2, smoff)) || (st = lbuf(&d_st, g.njobs)))
+                return st;
+            a.svs = d_svs;
+            a.jdev = d_jd;
+            a.z = d_z;
+            a.kidx = d_k;
+            a.kinh = d_kin;
+            a.sigL = d_sl;
+            a.sigR = d_sr;
+            a.wgt = d_w;
+            a.target = d_t;
+            a.outs = d_go;
+            a.nsel = d_ns;
+            a.smap = d_sm;
+            a.smap_off = d_so2;
+            a.status = d_st;
+            const int pb = panel_width(Np);
+            a.ws_stride = (size_t)(maxNz + 2) * Np * Np + (size_t)2 * pb * Np;
+            const Shape shp = shape_for(Np);
+            const void *kfn = shp == SHAPE_640 ? (const void *)k_diagonal<640, 32, true, true>
+                            : shp == SHAPE_1024_TWO ? (const void *)k_diagonal<1024, 32, true, true>
+                                                    : (const void *)k_diagonal<1024, 16, false, true>;
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, threads_for_shape(Np), 0) != hipSuccess ||
+                per_cu < 1)
+                per_cu = 1;
+            const double share = wsum > 0 ? work[gi] / wsum : 1.0;
+            long slots = std::max(1L, std::lround(share * m->n_cu * per_cu));
+            if (free_b > 0) {
+                const long fit = (long)(share * 0.7 * (double)free_b / (double)(a.ws_stride * sizeof(cd)));
+                slots = std::min(slots, std::max(1L, fit));
+            }
+            g.slots = (int)std::min<long>(slots, g.njobs);
+            // longest job first onto the least-loaded workgroup
+            std::vector<int> order(g.njobs);
+            for (int j = 0; j < g.njobs; ++j) order[j] = j;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+            typedef std::pair<double, int> Load;
+            std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+            for (int b = 0; b < g.slots; ++b) heap.push(Load(0.0, b));
+            std::vector<std::vector<int>> lists(g.slots);
+            for (int j : order) {
+                Load l = heap.top();
+                heap.pop();
+                lists[l.second].push_back(j);
+                heap.push(Load(l.first + cost[j], l.second));
+            }
+            std::vector<int> soff(1, 0), sjob;
+            for (auto &l : lists) {
+                sjob.insert(sjob.end(), l.begin(), l.end());
+                soff.push_back((int)sjob.size());
+            }
+            int *d_soff, *d_sjob;
+            cd *d_ws;
+            if ((st = lup(&d_soff, soff)) || (st = lup(&d_sjob, sjob)) || (st = lbuf(&d_ws, a.ws_stride * g.slots)))
+                return st;
+            a.sched_off = d_soff;
+            a.sched_job = d_sjob;
+            a.ws = d_ws;
+        }
+        groups.push_back(g);
+    }
+    HIPCHK(hipStreamSynchronize(m->s0));
+    HIPCHK(hipStreamBeginCapture(m->s0, hipStreamCaptureModeThreadLocal));
+    bool ok = hipEventRecord(m->fork, m->s0) == hipSuccess;
+    for (int gi = 0; gi < ng && ok; ++gi) {
+        const LGroup &g = groups[gi];
+        if (g.njobs == 0) continue;
+        hipStream_t gsm = m->gs[gi];
+        ok = hipStreamWaitEvent(gsm, m->fork, 0) == hipSuccess;
+        if (!ok) break;
+        for (int d : g.devs)
+            if (nout[d]) hipLaunchKernelGGL(k_zero_cd, dim3(256), dim3(256), 0, gsm, d_out[d], nout[d]);
+        const int th = threads_for_shape(g.Np);
+        switch (shape_for(g.Np)) {
+        case SHAPE_640:
+            hipLaunchKernelGGL((k_diagonal<640, 32, true, true>), dim3(g.slots), dim3(th), 0, gsm, g.a);
+            break;
+        case SHAPE_1024_TWO:
+            hipLaunchKernelGGL((k_diagonal<1024, 32, true, true>), dim3(g.slots), dim3(th), 0, gsm, g.a);
+            break;
+        default:
+            hipLaunchKernelGGL((k_diagonal<1024, 16, false, true>), dim3(g.slots), dim3(th), 0, gsm, g.a);
+        }
+        ok = hipGetLastError() == hipSuccess && hipEventRecord(m->gev[gi], gsm) == hipSuccess &&
+             hipStreamWaitEvent(m->s0, m->gev[gi], 0) == hipSuccess;
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(m->s0, &graph);
+    if (!ok || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(-12, "hipGraph capture of the uneven LDOS batch failed");
+    }
+    m->lgraph = graph;
+    HIPCHK(hipGraphInstantiate(&m->lexec, m->lgraph, nullptr, nullptr, 0));
+    HIPCHK(hipEventRecord(m->ev0, m->s0));
+    HIPCHK(hipGraphLaunch(m->lexec, m->s0));
+    HIPCHK(hipEventRecord(m->ev1, m->s0));
+    HIPCHK(hipEventSynchronize(m->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    m->last_s = ms * 1e-3;
+    int status = 0;
+    for (const LGroup &g : groups) {
+        if (g.njobs == 0) continue;
+        std::vector<int> h_st(g.njobs);
+        HIPCHK(hipMemcpy(h_st.data(), g.a.status, g.njobs * sizeof(int), hipMemcpyDeviceToHost));
+        for (int v : h_st)
+            if (v > 0 && !status) status = v;
+    }
+    for (int d = 0; d < ndev; ++d)
+        if (nout[d]) HIPCHK(hipMemcpy(ldos[d], d_out[d], nout[d] * sizeof(cd), hipMemcpyDeviceToHost));
+    return status;
 }
 
 int negf_multi_replay(negf_multi *m, double *const *tau) {

@@ -504,6 +504,27 @@ class Multi:
             self._ptrs(self._kw), iptr(nk), self._ptrs(self._tau)), "negf_multi_transmission")
         return [t.copy() for t in self._tau]
 
+    def ldos(self, energies, slices, kpoints=None):
+        """LDOS blocks of every device at the given slices (0-based, the same number per
+        device): one complex array (N, N, len(slices[d]), len(energies[d])) per device,
+        equal to NegfDevice.ldos_matrix restricted to those slices."""
+        n = len(self.devices)
+        E = [np.ascontiguousarray(e, dtype=np.float64) for e in energies]
+        kps = [NegfDevice._kp(None if kpoints is None else kpoints[d]) for d in range(n)]
+        kx, ky, kw = ([k[i] for k in kps] for i in range(3))
+        sl = [np.ascontiguousarray(s, dtype=np.int32) for s in slices]
+        nsel = len(sl[0])
+        if any(len(s) != nsel for s in sl):
+            raise ValueError("the same number of slices per device")
+        out = [np.zeros((d.N, d.N, nsel, len(e)), dtype=np.complex128, order="F") for d, e in zip(self.devices, E)]
+        nE = np.array([len(e) for e in E], dtype=np.int32)
+        nk = np.array([len(k) for k in kx], dtype=np.int32)
+        sptr = (ctypes.POINTER(ctypes.c_int) * n)(*[iptr(s) for s in sl])
+        self.last_status = check(self._lib.negf_multi_ldos(
+            self._h, self._ptrs(E), iptr(nE), self._ptrs(kx), self._ptrs(ky), self._ptrs(kw), iptr(nk), nsel, sptr,
+            self._ptrs(out)), "negf_multi_ldos")
+        return out
+
     def replay(self):
         """Launch the last call's hipGraph again (after set_potential on any device)."""
         if self._tau is None:

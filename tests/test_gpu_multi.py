@@ -7,7 +7,9 @@ assertion:
 - with the VALU variant forced, and with the autotuned choice, every device matches the
   CPU oracle to relative 1e-10;
 - graph replays after a potential update equal eager single-device runs with the new
-  potential, on every replay.
+  potential, on every replay;
+- LDOS at selected slices (negf_multi_ldos) equals each device's negf_ldos_batch at those
+  slices to 1e-12 relative (k-point sums by atomics) and the oracle to 1e-10.
 """
 import os
 
@@ -122,6 +124,23 @@ def test_multi_replay_after_potential_update(ng, variant):
             assert np.array_equal(got[i], first[i])
 
 
+def test_multi_ldos_selected_slices(ng):
+    """LDOS blocks at three slices per device in one graph, against each device's own
+    negf_ldos_batch (relative 1e-12: the k-point sums use atomics) and, for one device,
+    against the oracle (1e-10)."""
+    systems, devs, energies, kps = small_set(ng)
+    sl = [[0, s.N_z // 2, s.N_z - 1] for s in systems]
+    E = [e[:4] for e in energies]
+    m = ng.Multi(devs)
+    got = m.ldos(E, sl, kps)
+    assert m.last_status == 0
+    for d, e, g, kp, slc in zip(devs, E, got, kps, sl):
+        ref = d.ldos_matrix(e, kp)[:, :, slc, :]
+        assert np.max(np.abs(g - ref)) <= 1e-12 * np.abs(ref).max()
+    ref0 = np.moveaxis(O.ldos_matrix(systems[0], E[0][0]), 0, 2)[:, :, sl[0]]
+    assert np.linalg.norm(got[0][:, :, :, 0] - ref0) <= RTOL * np.linalg.norm(ref0)
+
+
 def test_multi_large_block_valu_path(ng):
     """N > 512 (padded 16-wide, one thread per row, the only variant there) next to a
     small device, against the oracle."""
@@ -162,3 +181,11 @@ def test_multi_survey_sizes_vs_single(ng, variant):
         np.testing.assert_allclose(g, ref, rtol=RTOL, atol=1e-13 * max(1.0, np.abs(ref).max()))
         if ng.Multi.variant(-(-d.N // 32) * 32) == "mfma":
             assert np.array_equal(g, ref)
+    # LDOS at 8 slices per device, two energies, against each device's own LDOS
+    sl = [np.linspace(0, d.N_z - 1, 8).round().astype(int) for d in devs]
+    E2 = [e[3:5] for e in energies]
+    L = m.ldos(E2, sl)
+    assert m.last_status == 0
+    for d, e, g, slc in zip(devs, E2, L, sl):
+        ref = d.ldos_matrix(e)[:, :, slc, :]
+        assert np.max(np.abs(g - ref)) <= 1e-12 * np.abs(ref).max()
