@@ -186,6 +186,17 @@ double negf_last_kernel_seconds(negf_ctx *ctx);
 int negf_last_fallbacks(const negf_ctx *ctx);
 
 /*
+ * NonLocalPotential_RealToPlanewave (src/grid.f90:137-234; the reference never calls it):
+ *   vrec(p, q, p', q') = (N_x N_y)^-2 sum V(x, y, x', y')
+ *                        exp(-2 pi i (p x / N_x + q y / N_y - p' x' / N_x - q' y' / N_y))
+ * for p, p' in -Rx..Rx and q, q' in -Ry..Ry.  V_real: N_x x N_y x N_x x N_y complex128,
+ * column-major; vrec: (2Rx+1) x (2Ry+1) x (2Rx+1) x (2Ry+1) complex128, column-major,
+ * index 0 = frequency -R.  The reference's check 2R <= N is kept; at |f| = N/2 the
+ * reference reads past its work array, here the periodic value is returned.
+ */
+int negf_nonlocal_vhat_from_real(int N_x, int N_y, const double *V_real, int Rx, int Ry, double *vrec);
+
+/*
  * Extension, no reference counterpart: replace V_reciprocal_all of a context (same NGX,
  * NGY, N_z), e.g. one step of a self-consistent potential loop.  Synchronous.
  */

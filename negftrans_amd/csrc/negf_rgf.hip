@@ -806,6 +806,37 @@ __global__ __launch_bounds__(256) void k_pw_to_real(int N, const int *nx, const 
 //   V(p, q) = (1/(Nx Ny)) sum_{x,y} V(x, y) e^{-2 pi i (p x/Nx + q y/Ny)}, p = 0..NGX,
 // p < 0 by conjugate symmetry, copied into the window |p| <= hx, |q| <= hy (zero elsewhere).
 // One workgroup per slice; twx/twy hold e^{-2 pi i j/N}.
+// One window-restricted DFT pass of the non-local transform: the input has column-major
+// dims d[0..3]; along `axis` (length N) the output keeps frequencies f = -R..R,
+//     out[.., f + R, ..] = scale * sum_n in[.., n, ..] tw[(f n) mod N],
+// tw[k] = exp(+-2 pi i k / N).  One thread per output entry (grid-stride).
+__global__ __launch_bounds__(256) void k_dft_axis(const cd *in, cd *out, int d0, int d1, int d2, int d3, int axis,
+                                                  int R, const cd *tw, double scale) {
+    const int din[4] = {d0, d1, d2, d3};
+    int dout[4] = {d0, d1, d2, d3};
+    dout[axis] = 2 * R + 1;
+    const int N = din[axis];
+    const size_t str[4] = {1, (size_t)d0, (size_t)d0 * d1, (size_t)d0 * d1 * d2};
+    const size_t total = (size_t)dout[0] * dout[1] * dout[2] * dout[3];
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        size_t rest = e, base = 0;
+        int f = 0;
+        for (int a = 0; a < 4; ++a) {
+            const int o = (int)(rest % dout[a]);
+            rest /= dout[a];
+            if (a == axis) f = o - R;
+            else base += (size_t)o * str[a];
+        }
+        cd acc = cmk(0.0, 0.0);
+        for (int n = 0; n < N; ++n) {
+            int k = (int)(((long long)f * n) % N);
+            if (k < 0) k += N;
+            acc = cfma(in[base + (size_t)n * str[axis]], tw[k], acc);
+        }
+        out[e] = cmk(acc.x * scale, acc.y * scale);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_vhat(int Nx, int Ny, const double *V, int NGX, int NGY,
                                               int hx, int hy, const cd *twx, const cd *twy,
                                               cd *scratch, cd *vrec) {
@@ -2010,6 +2041,44 @@ int negf_pw_to_real(int N, const int *nx_grid, const int *ny_grid, int nmat, con
                        dty, dS, dO);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(out, dO, g * nmat * sizeof(cd), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// NonLocalPotential_RealToPlanewave (src/grid.f90:137-234): four window-restricted DFT
+// passes -- backward over x' and y', forward over x and y -- with the reference's
+// (-1)^(i+j-ii-jj) shift and 1/(N_x N_y)^2 folded into the frequencies and the scale.
+int negf_nonlocal_vhat_from_real(int N_x, int N_y, const double *V_real, int Rx, int Ry, double *vrec) {
+    if (N_x < 1 || N_y < 1 || Rx < 0 || Ry < 0 || !V_real || !vrec)
+        return fail(-2, "bad arguments to negf_nonlocal_vhat_from_real");
+    // src/grid.f90:157-161
+    if (2 * Rx > N_x || 2 * Ry > N_y) return fail(-3, "size of V_reciprocal minus one larger than V_real");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(-7, "no GPU visible");
+    std::vector<cd> tpx, tpy, tmx, tmy;
+    twiddles(N_x, +1.0, tpx);
+    twiddles(N_y, +1.0, tpy);
+    twiddles(N_x, -1.0, tmx);
+    twiddles(N_y, -1.0, tmy);
+    const int fx = 2 * Rx + 1, fy = 2 * Ry + 1;
+    const size_t n0 = (size_t)N_x * N_y * N_x * N_y;
+    const size_t n1 = (size_t)N_x * N_y * fx * N_y, n2 = (size_t)N_x * N_y * fx * fy;
+    const size_t n3 = (size_t)fx * N_y * fx * fy, n4 = (size_t)fx * fy * fx * fy;
+    Temps tmp;
+    cd *dV, *d1, *d2, *d3, *d4, *dpx, *dpy, *dmx, *dmy;
+    int st;
+    if ((st = tmp.upload(&dV, reinterpret_cast<const cd *>(V_real), n0, nullptr)) ||
+        (st = tmp.upload(&dpx, tpx.data(), N_x, nullptr)) || (st = tmp.upload(&dpy, tpy.data(), N_y, nullptr)) ||
+        (st = tmp.upload(&dmx, tmx.data(), N_x, nullptr)) || (st = tmp.upload(&dmy, tmy.data(), N_y, nullptr)) ||
+        (st = tmp.alloc(&d1, n1)) || (st = tmp.alloc(&d2, n2)) || (st = tmp.alloc(&d3, n3)) || (st = tmp.alloc(&d4, n4)))
+        return st;
+    auto grid = [](size_t n) { return dim3((unsigned)std::min<size_t>((n + 255) / 256, 65535)); };
+    const double sc = 1.0 / ((double)N_x * N_y * (double)N_x * N_y);
+    hipLaunchKernelGGL(k_dft_axis, grid(n1), dim3(256), 0, nullptr, dV, d1, N_x, N_y, N_x, N_y, 2, Rx, dpx, 1.0);
+    hipLaunchKernelGGL(k_dft_axis, grid(n2), dim3(256), 0, nullptr, d1, d2, N_x, N_y, fx, N_y, 3, Ry, dpy, 1.0);
+    hipLaunchKernelGGL(k_dft_axis, grid(n3), dim3(256), 0, nullptr, d2, d3, N_x, N_y, fx, fy, 0, Rx, dmx, 1.0);
+    hipLaunchKernelGGL(k_dft_axis, grid(n4), dim3(256), 0, nullptr, d3, d4, fx, N_y, fx, fy, 1, Ry, dmy, sc);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(vrec, d4, n4 * sizeof(cd), hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -6,7 +6,7 @@ module grid
     use negf_gpu_c
     implicit none
     private
-    public LocalPotential_RealToPlanewave, PlaneWaveBasis_construction_findsize, &
+    public LocalPotential_RealToPlanewave, NonLocalPotential_RealToPlanewave, PlaneWaveBasis_construction_findsize, &
         Greensfunction_PlanewaveToReal, PlaneWaveBasis_construction, Hamiltonian_construction, &
         E_minus_H_construction, Kpoint_mesh_construction, Transmission_solver
 contains
@@ -36,6 +36,33 @@ contains
         call negf_check(negf_vhat_from_real(N_x, N_y, 1, V, ngx, ngy, 1, out), "negf_vhat_from_real")
         V_reciprocal = out
     end subroutine LocalPotential_RealToPlanewave
+
+    ! V_reciprocal(p, q, p', q') = (N_x N_y)^-2 sum V(x, y, x', y') exp(-2 pi i (p x/N_x + q y/N_y
+    ! - p' x'/N_x - q' y'/N_y)) over the whole symmetric window, src/grid.f90:137-234 (the
+    ! reference's validation messages kept).
+    subroutine NonLocalPotential_RealToPlanewave(V_real, V_reciprocal)
+        complex*16, intent(in) :: V_real(:, :, :, :)
+        complex*16, allocatable, intent(inout) :: V_reciprocal(:, :, :, :)
+        complex(c_double_complex), allocatable :: V(:, :, :, :), out(:, :, :, :)
+        integer :: N_x, N_y, rx, ry
+        N_x = size(V_real, 1)
+        N_y = size(V_real, 2)
+        if (size(V_reciprocal, 1) - 1 > N_x .or. size(V_reciprocal, 2) - 1 > N_y) then
+            write(0, *) "ERROR: 'The size of V_reciprocal minus one' should not be larger than 'the size of V_real'"
+            error stop 1
+        end if
+        if (-lbound(V_reciprocal, 1) /= ubound(V_reciprocal, 1) .or. &
+            -lbound(V_reciprocal, 2) /= ubound(V_reciprocal, 2)) then
+            write(0, *) "ERROR: The index boundary of V_reciprocal must be symmetric about zero"
+            error stop 1
+        end if
+        rx = ubound(V_reciprocal, 1)
+        ry = ubound(V_reciprocal, 2)
+        allocate(V(N_x, N_y, N_x, N_y), out(-rx:rx, -ry:ry, -rx:rx, -ry:ry))
+        V = V_real
+        call negf_check(negf_nonlocal_vhat_from_real(N_x, N_y, V, rx, ry, out), "negf_nonlocal_vhat_from_real")
+        V_reciprocal = out
+    end subroutine NonLocalPotential_RealToPlanewave
 
     ! Real-space diagonal of one plane-wave block (src/grid.f90:71-135).
     subroutine Greensfunction_PlanewaveToReal(subGreenFunc, nx_grid, ny_grid, subGreenDiag)

@@ -85,6 +85,13 @@ module negf_gpu_c
             complex(c_double_complex) :: vrec(*)
         end function
 
+        integer(c_int) function negf_nonlocal_vhat_from_real(N_x, N_y, V_real, Rx, Ry, vrec) &
+                bind(C, name="negf_nonlocal_vhat_from_real")
+            import :: c_int, c_double_complex
+            integer(c_int), value :: N_x, N_y, Rx, Ry
+            complex(c_double_complex) :: V_real(*), vrec(*)
+        end function
+
         integer(c_int) function negf_inverse_batch(N, count, A, inv) bind(C, name="negf_inverse_batch")
             import :: c_int, c_double_complex
             integer(c_int), value :: N, count

@@ -239,6 +239,20 @@ def local_potential_real_to_planewave(V_real, NGX, NGY, full_window=False):
     return out
 
 
+def nonlocal_potential_real_to_planewave(V_real, Rx, Ry):
+    """NonLocalPotential_RealToPlanewave (src/grid.f90:137-234) on the GPU:
+    V_real(N_x, N_y, N_x, N_y) complex -> V_reciprocal(-Rx:Rx, -Ry:Ry, -Rx:Rx, -Ry:Ry),
+    returned as a Fortran-ordered array indexed [p + Rx, q + Ry, p' + Rx, q' + Ry]."""
+    V = np.asfortranarray(np.asarray(V_real, dtype=np.complex128))
+    N_x, N_y = V.shape[:2]
+    if V.shape != (N_x, N_y, N_x, N_y):
+        raise ValueError("V_real must be (N_x, N_y, N_x, N_y)")
+    out = np.zeros((2 * Rx + 1, 2 * Ry + 1, 2 * Rx + 1, 2 * Ry + 1), dtype=np.complex128, order="F")
+    check(_lib.load().negf_nonlocal_vhat_from_real(N_x, N_y, dptr(V), Rx, Ry, dptr(out)),
+          "negf_nonlocal_vhat_from_real")
+    return out
+
+
 def inverse(M):
     """Batched drop-in for inverse (src/math_kernel.f90:8-34). M: (N, N) or (N, N, B)."""
     A = np.asfortranarray(np.asarray(M, dtype=np.complex128))

@@ -158,6 +158,30 @@ def local_potential_r2p(V_slice: np.ndarray, NGX: int, NGY: int) -> np.ndarray:
     return out
 
 
+def nonlocal_potential_r2p(V_real: np.ndarray, Rx: int, Ry: int) -> np.ndarray:
+    """NonLocalPotential_RealToPlanewave (src/grid.f90:137-234), never called by the
+    reference program.  V_real(N_x, N_y, N_x, N_y) complex, Fortran index order
+    (x, y, x', y').  The steps of the reference: multiply by (-1)^(i+j-ii-jj)
+    (:171-179), unnormalised backward 2-D DFT over (x', y') (:184-201), forward 2-D DFT
+    over (x, y) (:203-219), divide by (N_x N_y)^2 and keep frequencies -R..R of every
+    index, frequency f at 0-based position f + N/2 (:222-232).  Returns an array
+    indexed [p + Rx, q + Ry, p' + Rx, q' + Ry].  At |f| = N/2 the reference reads past
+    its work array (index N + 1); this restatement takes the periodic value."""
+    N_x, N_y = V_real.shape[:2]
+    if 2 * Rx > N_x or 2 * Ry > N_y:
+        raise ValueError("size of V_reciprocal minus one larger than size of V_real")
+    x = np.arange(N_x)[:, None, None, None]
+    y = np.arange(N_y)[None, :, None, None]
+    xp = np.arange(N_x)[None, None, :, None]
+    yp = np.arange(N_y)[None, None, None, :]
+    W = V_real * (-1.0) ** (x + y - xp - yp)
+    W = np.fft.ifft2(W, axes=(2, 3)) * (N_x * N_y)
+    W = np.fft.fft2(W, axes=(0, 1))
+    ix = (np.arange(-Rx, Rx + 1) + N_x // 2) % N_x
+    iy = (np.arange(-Ry, Ry + 1) + N_y // 2) % N_y
+    return W[np.ix_(ix, iy, ix, iy)] / float(N_x * N_y) ** 2
+
+
 def v_reciprocal_all(V_real: np.ndarray, NGX: int, NGY: int, full_window: bool = False):
     """The V_reciprocal_all(-NGX:NGX, -NGY:NGY, N_z) table built by the reference's
     main program (src/main.f90:217-249): per slice the transform above, but only

@@ -281,3 +281,16 @@ def test_pw_to_real(ng):
     got = ng.greensfunction_planewave_to_real(M, nx, ny, 64, 64)
     for b in range(2):
         assert rel_err(got[:, :, b], O.pw_to_real(M[:, :, b], nx, ny, 64, 64)) < RTOL
+
+
+# ------------------------------------------------------------------ f4: unused reference path
+@pytest.mark.parametrize("Nx,Ny,Rx,Ry", [(6, 4, 2, 1), (16, 12, 5, 4), (8, 8, 4, 4), (32, 32, 8, 8)])
+def test_nonlocal_potential(ng, Nx, Ny, Rx, Ry):
+    """NonLocalPotential_RealToPlanewave (src/grid.f90:137-234) on the GPU against the
+    oracle (its FFT restatement, pinned by the literal four-fold sum and the local-potential
+    identity), including the |f| = N/2 edge (periodic value)."""
+    rng = np.random.default_rng(Nx * 7 + Ny)
+    V = rand_c(rng, Nx, Ny, Nx, Ny)
+    got = ng.nonlocal_potential_real_to_planewave(V, Rx, Ry)
+    ref = O.nonlocal_potential_r2p(V, Rx, Ry)
+    assert rel_err(got, ref) < RTOL

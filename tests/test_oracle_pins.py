@@ -464,3 +464,43 @@ def test_lead_decimation_uniform_wire_transmits_one_per_open_channel():
         T = O.transmission(sd, [E])[0]
         assert abs(T - n_open) <= 1e-5 * max(n_open, 1)
         assert abs(O.transmission(s, [E])[0] - n_open) > 4e-4
+
+
+def test_nonlocal_potential_literal_sum():
+    """NonLocalPotential_RealToPlanewave restated through FFTs against the literal four-fold
+    sum (1/(N_x N_y)^2) sum V(x,y,x',y') exp(-2 pi i (p x/N_x + q y/N_y - p' x'/N_x - q' y'/N_y))."""
+    Nx, Ny, Rx, Ry = 6, 4, 2, 1
+    r = rng(31)
+    V = r.standard_normal((Nx, Ny, Nx, Ny)) + 1j * r.standard_normal((Nx, Ny, Nx, Ny))
+    got = O.nonlocal_potential_r2p(V, Rx, Ry)
+    x = np.arange(Nx)[:, None, None, None]
+    y = np.arange(Ny)[None, :, None, None]
+    xp = np.arange(Nx)[None, None, :, None]
+    yp = np.arange(Ny)[None, None, None, :]
+    for p in range(-Rx, Rx + 1):
+        for q in range(-Ry, Ry + 1):
+            for pp in range(-Rx, Rx + 1):
+                for qq in range(-Ry, Ry + 1):
+                    ph = np.exp(-2j * np.pi * (p * x / Nx + q * y / Ny - pp * xp / Nx - qq * yp / Ny))
+                    want = np.sum(V * ph) / (Nx * Ny) ** 2
+                    assert abs(got[p + Rx, q + Ry, pp + Rx, qq + Ry] - want) < 1e-13
+
+
+def test_nonlocal_of_a_local_potential():
+    """V(r, r') = delta(r - r') v(r): the non-local transform is the local one of the
+    difference frequency, V(p,q,p',q') = v_hat(p - p', q - q') / (N_x N_y), with v_hat from
+    local_potential_r2p (pinned by its own closed forms)."""
+    Nx, Ny, R = 10, 10, 2      # the local window +-4 needs NGY + 1 + N_y / 2 <= N_y
+    v = rng(32).standard_normal((Nx, Ny))
+    V = np.zeros((Nx, Ny, Nx, Ny), dtype=np.complex128)
+    for i in range(Nx):
+        for j in range(Ny):
+            V[i, j, i, j] = v[i, j]
+    got = O.nonlocal_potential_r2p(V, R, R)
+    loc = O.local_potential_r2p(v, 2 * R, 2 * R)
+    for p in range(-R, R + 1):
+        for q in range(-R, R + 1):
+            for pp in range(-R, R + 1):
+                for qq in range(-R, R + 1):
+                    want = loc[p - pp + 2 * R, q - qq + 2 * R] / (Nx * Ny)
+                    assert abs(got[p + R, q + R, pp + R, qq + R] - want) < 1e-14
