@@ -126,20 +126,22 @@ __device__ __forceinline__ void row_product(cd *__restrict__ Rn, const cd *__res
 
 // TWO: two lanes per matrix row and the trailing update on the FP64 matrix cores
 // (gje_invert4, Np <= 512); otherwise one thread per row (gje_invert2).
-template <int PB, bool TWO>
-using GjeSh = typename std::conditional<TWO, Gje4Shared<PB>, GjeShared<PB>>::type;
+template <int PB, bool TWO, int MAXNP = NEGF_MAXNP4, int TCH = NEGF_TCH>
+using GjeSh = typename std::conditional<TWO, Gje4Shared<PB, double, MAXNP, TCH>, GjeShared<PB>>::type;
 
-template <int PB, bool TWO>
-__device__ __forceinline__ int gje_any(cd *Z, int Np, GjeSh<PB, TWO> &s, cd *scratch) {
-    if constexpr (TWO) return gje_invert4<PB>(Z, Np, s, scratch);
+template <int PB, bool TWO, class SH>
+__device__ __forceinline__ int gje_any(cd *Z, int Np, SH &s, cd *scratch) {
+    if constexpr (TWO) return gje_invert4<PB, double, SH>(Z, Np, s, scratch);
     else return gje_invert2<PB>(Z, Np, s, scratch);
 }
 
 // MULTI: uneven batch -- jobs from a.sched_job, each with its device's SysView (a.svs);
 // otherwise every job uses a.sv and workgroup b runs jobs b, b + gridDim.x, ...
-template <int MAXT, int PB, bool TWO, bool MULTI = false>
+// MAXNP / TCH: size limits of the Gauss-Jordan shared state (smaller for small Np, so
+// that several workgroups fit on a CU).
+template <int MAXT, int PB, bool TWO, bool MULTI = false, int MAXNP = NEGF_MAXNP4, int TCH = NEGF_TCH>
 __global__ __launch_bounds__(MAXT) void k_transmission(TransArgs a) {
-    __shared__ GjeSh<PB, TWO> s;
+    __shared__ GjeSh<PB, TWO, MAXNP, TCH> s;
     __shared__ RowProdShared rs;
     const int Np = a.sv.Np, ldR = a.nLmax;
     const size_t NN = (size_t)Np * Np;
@@ -1195,6 +1197,26 @@ int slots_for(negf_ctx *c, const void *kernel, int threads, int njobs, size_t pe
 
 }  // namespace
 
+// FP64 transmission kernel for padded size Np: the matrix-core Gauss-Jordan with its
+// shared state sized for Np <= 64 (5 workgroups per CU), Np <= 128 (3 per CU) or up to
+// 512 (1 per CU), and the one-thread-per-row form above 512.
+template <bool MULTI>
+static const void *trans_kernel(int Np) {
+    if (Np <= 64) return (const void *)k_transmission<128, 32, true, MULTI, 64, 32>;
+    if (Np <= 128) return (const void *)k_transmission<256, 32, true, MULTI, 128, 64>;
+    if (Np <= 320) return (const void *)k_transmission<640, 32, true, MULTI>;
+    if (Np <= 512) return (const void *)k_transmission<1024, 32, true, MULTI>;
+    return (const void *)k_transmission<1024, 16, false, MULTI>;
+}
+template <bool MULTI>
+static void launch_trans(int Np, int slots, int threads, hipStream_t st, const TransArgs &a) {
+    if (Np <= 64) hipLaunchKernelGGL((k_transmission<128, 32, true, MULTI, 64, 32>), dim3(slots), dim3(threads), 0, st, a);
+    else if (Np <= 128) hipLaunchKernelGGL((k_transmission<256, 32, true, MULTI, 128, 64>), dim3(slots), dim3(threads), 0, st, a);
+    else if (Np <= 320) hipLaunchKernelGGL((k_transmission<640, 32, true, MULTI>), dim3(slots), dim3(threads), 0, st, a);
+    else if (Np <= 512) hipLaunchKernelGGL((k_transmission<1024, 32, true, MULTI>), dim3(slots), dim3(threads), 0, st, a);
+    else hipLaunchKernelGGL((k_transmission<1024, 16, false, MULTI>), dim3(slots), dim3(threads), 0, st, a);
+}
+
 static void launch_inverse(int Np, int count, hipStream_t st, int N, const cd *A, const cd *Sub, cd *work,
                            cd *inv, int *status) {
     const int th = threads_for_shape(Np);
@@ -1487,9 +1509,7 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
     }
     const size_t per_slot =
         ((size_t)2 * c->Np * c->Np + (size_t)2 * nLmax * c->Np + (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
-    const void *kfn = shp == SHAPE_640 ? (const void *)k_transmission<640, 32, true>
-                    : shp == SHAPE_1024_TWO ? (const void *)k_transmission<1024, 32, true>
-                                            : (const void *)k_transmission<1024, 16, false>;
+    const void *kfn = trans_kernel<false>(c->Np);
     const int slots = slots_for(c, kfn, threads, nj, per_slot);
     if ((st = ensure_ws(c, per_slot * slots))) return st;
     a.z = d_z;
@@ -1509,9 +1529,7 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
     a.tau = d_tau;
     a.status = d_st;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    if (shp == SHAPE_640) hipLaunchKernelGGL((k_transmission<640, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
-    else if (shp == SHAPE_1024_TWO) hipLaunchKernelGGL((k_transmission<1024, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
-    else hipLaunchKernelGGL((k_transmission<1024, 16, false>), dim3(slots), dim3(threads), 0, c->stream, a);
+    launch_trans<false>(c->Np, slots, threads, c->stream, a);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     return finish_transmission(c, nj, d_tau, d_st, je, jk, kw, tau, tau_jobs);
@@ -2128,14 +2146,12 @@ int variant_threads(int v, int Np) { return v == VAR_VALU ? std::min(1024, round
 int variant_pb(int v) { return v == VAR_VALU ? 16 : 32; }
 const void *variant_kernel(int v, int Np) {
     if (v == VAR_VALU) return (const void *)k_transmission<1024, 16, false, true>;
-    return Np <= 320 ? (const void *)k_transmission<640, 32, true, true>
-                     : (const void *)k_transmission<1024, 32, true, true>;
+    return trans_kernel<true>(Np);
 }
 void launch_variant(int v, int Np, int slots, hipStream_t st, const TransArgs &a) {
     const int th = variant_threads(v, Np);
     if (v == VAR_VALU) hipLaunchKernelGGL((k_transmission<1024, 16, false, true>), dim3(slots), dim3(th), 0, st, a);
-    else if (Np <= 320) hipLaunchKernelGGL((k_transmission<640, 32, true, true>), dim3(slots), dim3(th), 0, st, a);
-    else hipLaunchKernelGGL((k_transmission<1024, 32, true, true>), dim3(slots), dim3(th), 0, st, a);
+    else launch_trans<true>(Np, slots, th, st, a);
 }
 
 // Variant picked per Np by the probe of tune_group, shared by every negf_multi.

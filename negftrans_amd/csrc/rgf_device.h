@@ -538,7 +538,6 @@ __device__ __forceinline__ cd lane_swap(cd v) { return cmk(lane_swap(v.x), lane_
 // sits in P[t][0] (no runtime register indexing).
 // ====================================================================================
 #define NEGF_MAXNP4 512
-#define NEGF_LDT (NEGF_TCH + 1)   // LDS row stride of the staged pivot rows (padded)
 
 typedef double negf_d4 __attribute__((ext_vector_type(4)));
 typedef float negf_f4 __attribute__((ext_vector_type(4)));
@@ -651,30 +650,35 @@ struct GjePrec<float> {
     }
 };
 
-template <int PB, class R = double>
+// MAXNP: largest padded size the workgroup serves (blockDim = 2 Np <= 2 MAXNP); TCH:
+// columns of the old pivot rows staged in LDS at a time.  Smaller values shrink the LDS
+// footprint so that several workgroups share a CU at small Np; the arithmetic does not
+// depend on either.
+template <int PB, class R = double, int MAXNP = NEGF_MAXNP4, int TCH = NEGF_TCH>
 struct Gje4Shared {
     typedef typename GjePrec<R>::C C;
-    int perm[NEGF_MAXNP4];
-    int ipiv[NEGF_MAXNP4];
-    int sigma[NEGF_MAXNP4];
+    static constexpr int MAXW = 2 * MAXNP / 64;   // waves per workgroup at most
+    static constexpr int CH = TCH, LDT = TCH + 1;  // LDS row stride of the staged rows (padded)
+    int perm[MAXNP];
+    int ipiv[MAXNP];
+    int sigma[MAXNP];
     int disp[2 * PB];
     int ndisp;
     int flag;
-    R rv[2][NEGF_MAXW];
-    int ri[2][NEGF_MAXW];
-    C cand[2][NEGF_MAXW][PB];   // each wave's best candidate row (panel columns)
+    R rv[2][MAXW];
+    int ri[2][MAXW];
+    C cand[2][MAXW][PB];        // each wave's best candidate row (panel columns)
     C rowC[2][PB];              // row c (the pivot position) before the swap
-    C colJ[2][NEGF_MAXNP4];     // panel column J of every row
-    C Tl[PB * NEGF_LDT];        // old pivot rows of one column chunk: Tl[p * LDT + (c - c0)]
-    double red[NEGF_MAXW];
+    C colJ[2][MAXNP];           // panel column J of every row
+    C Tl[PB * LDT];             // old pivot rows of one column chunk: Tl[p * LDT + (c - c0)]
+    double red[MAXW];
 };
 
 // One pivot step, column J = 4 g + KJ of the panel; P[.][q] holds panel column
 // 4 ((q + g) % (PB / 4)) + k.
-template <int PB, int KJ, class R>
+template <int PB, int KJ, class R, class SH>
 __device__ __forceinline__ void gje4_step(typename GjePrec<R>::C (&P)[2][PB / 4], const int g, const int k0,
-                                          const int Np, const int w, const int j, const int k,
-                                          Gje4Shared<PB, R> &s) {
+                                          const int Np, const int w, const int j, const int k, SH &s) {
     typedef typename GjePrec<R>::C C;
     constexpr int KK = PB / 4;
     const int t = threadIdx.x, lane = t & 63, nw = blockDim.x >> 6;
@@ -781,9 +785,9 @@ __device__ __forceinline__ void gje4_step(typename GjePrec<R>::C (&P)[2][PB / 4]
 // In-place inversion of Z (Np x Np) by a workgroup of 2 Np threads; on return
 // inverse[:, c] = Z[:, s.sigma[c]].  scratch: (2 PB) x Np elements per workgroup.
 // Returns the 1-based column of the first exactly-zero pivot, or 0.
-template <int PB, class R = double>
-__device__ __forceinline__ int gje_invert4(typename GjePrec<R>::C *__restrict__ Z, const int Np,
-                                           Gje4Shared<PB, R> &s, typename GjePrec<R>::C *__restrict__ scratch) {
+template <int PB, class R = double, class SH = Gje4Shared<PB, R>>
+__device__ __forceinline__ int gje_invert4(typename GjePrec<R>::C *__restrict__ Z, const int Np, SH &s,
+                                           typename GjePrec<R>::C *__restrict__ scratch) {
     static_assert(PB == 32, "two 16-row tiles per wave hold one 32-column panel");
     typedef GjePrec<R> G;
     typedef typename G::C C;
@@ -870,7 +874,7 @@ __device__ __forceinline__ int gje_invert4(typename GjePrec<R>::C *__restrict__ 
             Z[s.disp[q] + col * ld] = Dg[q + (size_t)col * PB];
         }
         __syncthreads();
-        // trailing update on the matrix cores, one chunk of NEGF_TCH columns at a time;
+        // trailing update on the matrix cores, one chunk of SH::CH columns at a time;
         // the pivot rows themselves (this wave's rows when R0 == k0) start from zero.
         // Column tiles run in increasing order (panel tiles skipped) and the base values
         // of the next tile are loaded while the current one is on the matrix cores.
@@ -892,13 +896,13 @@ __device__ __forceinline__ int gje_invert4(typename GjePrec<R>::C *__restrict__ 
             }
         };
         load_base(k0 == 0 ? PB : 0);
-        for (int c0 = 0; c0 < Np; c0 += NEGF_TCH) {
-            const int c1 = min(c0 + NEGF_TCH, Np);
+        for (int c0 = 0; c0 < Np; c0 += SH::CH) {
+            const int c1 = min(c0 + SH::CH, Np);
             if (c0 >= k0 && c1 <= k0 + PB) continue;       // chunk is all panel
             for (int e = t; e < (c1 - c0) * PB; e += blockDim.x) {
                 const int pp = e % PB, cl = e / PB;
                 const int col = c0 + cl;
-                if (!(col >= k0 && col < k0 + PB)) s.Tl[pp * NEGF_LDT + cl] = Tg[e + (size_t)c0 * PB];
+                if (!(col >= k0 && col < k0 + PB)) s.Tl[pp * SH::LDT + cl] = Tg[e + (size_t)c0 * PB];
             }
             __syncthreads();
             for (int ct = c0; ct < c1; ct += 16) {
@@ -915,10 +919,10 @@ __device__ __forceinline__ int gje_invert4(typename GjePrec<R>::C *__restrict__ 
                 __builtin_amdgcn_sched_barrier(0);          // keep the prefetch ahead of the MFMAs
                 int lx = threadIdx.x & 63;
                 asm volatile("" : "+v"(lx));                 // recompute, do not keep live
-                const C *tl = s.Tl + (lx >> 4) * NEGF_LDT + (ct - c0 + (lx & 15));
+                const C *tl = s.Tl + (lx >> 4) * SH::LDT + (ct - c0 + (lx & 15));
 #pragma unroll
                 for (int q = 0; q < KK; ++q) {
-                    const C tv = tl[4 * q * NEGF_LDT];
+                    const C tv = tl[4 * q * SH::LDT];
                     const R ntim = -tv.y;
                     ar0 = G::mfma(tv.x, P[0][q].x, ar0);
                     ar1 = G::mfma(tv.x, P[1][q].x, ar1);
