@@ -17,6 +17,8 @@ Configurations (all single-device, k = 0):
             64 x 64 x 1000 grid of Gaussian-correlated disorder (sigma = 3 meV,
             correlation length 5 nm, periodic in x, y, seed 20261203), tapered to zero
             over 10 nm at both z ends; V_L = V_R = 0; 4096 energies in 0-30 meV.
+  uneven_c2()  48 devices, N in {61, 113, 197, 317, 509}, N_z in 200-1500, seeded
+            disorder of disorder_c1's kind each, 256 energies (the uneven batch).
   qpc_c4_instance(seed)  the learned adaptive grid's instances: qpc_c0's form with
             N = 113 (R^2 = 36.5), N_z = 400 slices of 0.25 nm on a 32 x 32 grid, seeded
             V0 in 6-14 meV, w in 6-10 nm, saddle centre 40-60 nm and width 8-12 nm,
@@ -143,6 +145,29 @@ def qpc_c4_instance(seed, n_energy=512, N_z=400, R2=36.5, grid=32) -> Workload:
                ETA=ms * 1e-6 / HARTREE, TRANSMISSION_GRID=(0.0, ms * 0.020, n_energy))
     E = uniform_energies(0.0, 0.020, n_energy)
     return Workload("qpc_c4", inp, ms * V, E, extra=dict(V0=V0, w=w, z0=z0, zw=zw, hw=hw))
+
+
+def uneven_c2(seed=20261203, n_devices=48, n_energy=256, sizes=(61, 113, 197, 317, 509), N_z_range=(200, 1500),
+              grid=64, sigma=0.003) -> list:
+    """The uneven batch: n_devices devices, N drawn from `sizes`, N_z uniform in N_z_range,
+    each with its own seeded disorder of disorder_c1's kind (sigma, 5 nm correlation,
+    tapered over 10 nm) on a grid x grid x N_z mesh with dz = 0.25 nm, n_energy energies
+    in 0-30 meV."""
+    R2 = {61: 18.5, 113: 36.5, 197: 64.5, 317: 100.5, 509: 162.5}
+    rng = np.random.default_rng(seed)
+    out = []
+    for d in range(n_devices):
+        N = int(rng.choice(sizes))
+        N_z = int(rng.integers(N_z_range[0], N_z_range[1] + 1))
+        Lx = Ly = 400.0
+        dz = 2.5
+        ms = MSTAR_GAAS
+        V = gaussian_disorder(grid, N_z, Lx, dz, sigma, 50.0, seed + 1000 + d, 100.0)
+        inp = dict(if_transmission=True, LX=Lx, LY=Ly, LZ=N_z * dz, NKX=1, NKY=1, ENCUT=encut_for(R2[N], Lx),
+                   ETA=ms * 1e-6 / HARTREE, TRANSMISSION_GRID=(0.0, ms * 0.030, n_energy))
+        out.append(Workload(f"uneven_c2_{d}", inp, ms * V, uniform_energies(0.0, 0.030, n_energy),
+                            extra=dict(N=N, N_z=N_z)))
+    return out
 
 
 def fourier_features(V_eV, shape=(16, 16, 8), scale=0.010 * MSTAR_GAAS):
