@@ -17,6 +17,8 @@ Configurations (all single-device, k = 0):
             64 x 64 x 1000 grid of Gaussian-correlated disorder (sigma = 3 meV,
             correlation length 5 nm, periodic in x, y, seed 20261203), tapered to zero
             over 10 nm at both z ends; V_L = V_R = 0; 4096 energies in 0-30 meV.
+  disorder_c3  disorder_c1's kind with N = 797 (R^2 = 256.5), N_z = 2000, sigma = 5 meV,
+            80 x 80 x 2000 grid, 1024 energies in 0-30 meV.
   uneven_c2()  48 devices, N in {61, 113, 197, 317, 509}, N_z in 200-1500, seeded
             disorder of disorder_c1's kind each, 256 energies (the uneven batch).
   qpc_c4_instance(seed)  the learned adaptive grid's instances: qpc_c0's form with
@@ -145,6 +147,19 @@ def qpc_c4_instance(seed, n_energy=512, N_z=400, R2=36.5, grid=32) -> Workload:
                ETA=ms * 1e-6 / HARTREE, TRANSMISSION_GRID=(0.0, ms * 0.020, n_energy))
     E = uniform_energies(0.0, 0.020, n_energy)
     return Workload("qpc_c4", inp, ms * V, E, extra=dict(V0=V0, w=w, z0=z0, zw=zw, hw=hw))
+
+
+def disorder_c3(n_energy=1024, N_z=2000, R2=256.5, grid=80, sigma=0.005, seed=20261203) -> Workload:
+    """disorder_c1's kind at the largest size: N = 797 (R^2 = 256.5), N_z = 2000 slices of
+    0.25 nm, sigma = 5 meV, on an 80 x 80 x N_z grid (the V-hat window of N = 797 needs
+    NGX = 35), 1024 energies in 0-30 meV."""
+    Lx = Ly = 400.0
+    dz = 2.5
+    ms = MSTAR_GAAS
+    V = gaussian_disorder(grid, N_z, Lx, dz, sigma, 50.0, seed, 100.0)
+    inp = dict(if_transmission=True, LX=Lx, LY=Ly, LZ=N_z * dz, NKX=1, NKY=1, ENCUT=encut_for(R2, Lx),
+               ETA=ms * 1e-6 / HARTREE, TRANSMISSION_GRID=(0.0, ms * 0.030, n_energy))
+    return Workload("disorder_c3", inp, ms * V, uniform_energies(0.0, 0.030, n_energy))
 
 
 def uneven_c2(seed=20261203, n_devices=48, n_energy=256, sizes=(61, 113, 197, 317, 509), N_z_range=(200, 1500),

@@ -294,3 +294,32 @@ def test_nonlocal_potential(ng, Nx, Ny, Rx, Ry):
     got = ng.nonlocal_potential_real_to_planewave(V, Rx, Ry)
     ref = O.nonlocal_potential_r2p(V, Rx, Ry)
     assert rel_err(got, ref) < RTOL
+
+
+def test_transmission_c3_block_size(ng):
+    """The largest configuration's block size (N = 797, padded 800: one thread per row,
+    16-column panels) on the disorder of that configuration: against the oracle on the
+    first 6 slices (2 energies, relative 1e-10), and at N_z = 40 the mirror property --
+    with V_L = V_R, reversing the potential in z leaves T unchanged (two-terminal
+    reciprocity; the tolerance allows the O(eta) loss of unitarity) -- with
+    0 <= T <= open channels."""
+    from negftrans_amd import workloads
+    wl = workloads.disorder_c3(N_z=40)
+    NGX, NGY = wl.ngx()
+    vrec = np.asfortranarray(ng.local_potential_real_to_planewave(wl.V_real_eV / O.HARTREE, NGX, NGY))
+    a = wl.atomic()
+    nx, ny = wl.basis()
+    assert len(nx) == 797
+    E = wl.energies_hartree()[[700, 1000]]       # 5 and 9 open channels
+    short = np.asfortranarray(vrec[:, :, :6])
+    a6 = ng.Atomic(V_L=a.V_L, V_R=a.V_R, ETA=a.ETA, LX=a.LX, LY=a.LY, LZ=a.LZ * 6 / 40, ENCUT=a.ENCUT)
+    got6 = ng.NegfDevice(nx, ny, a6, short, NGX, NGY).transmission(E)
+    s6 = O.System(O.Atomic(V_L=a.V_L, V_R=a.V_R, ETA=a.ETA, LX=a.LX, LY=a.LY, LZ=a.LZ * 6 / 40, ENCUT=a.ENCUT),
+                  nx, ny, np.asarray(short), NGX, NGY)
+    np.testing.assert_allclose(got6, O.transmission(s6, E), rtol=RTOL, atol=1e-12)
+    fwd = ng.NegfDevice(nx, ny, a, vrec, NGX, NGY).transmission(E)
+    rev = ng.NegfDevice(nx, ny, a, np.asfortranarray(vrec[:, :, ::-1]), NGX, NGY).transmission(E)
+    kin = O.kinetic(nx, ny, a.LX, a.LY, 0.0, 0.0)
+    n_open = np.array([np.sum(2 * e - kin > 0) for e in E])
+    assert np.all(fwd >= -1e-12) and np.all(fwd <= n_open + 1e-6) and n_open.min() >= 2
+    np.testing.assert_allclose(rev, fwd, rtol=1e-6, atol=1e-9)
