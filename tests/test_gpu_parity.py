@@ -220,6 +220,23 @@ def test_transmission_disorder_c1_full_size(ng):
         assert got[i] == pytest.approx(ref, rel=RTOL, abs=1e-12)
 
 
+def test_density_c1_full_size(ng):
+    """D mode at the metric configuration's size (N = 317, N_z = 1000): one contour node
+    of the density (z in the upper half plane, weight 1) against the oracle's D-mode
+    restatement on every slice, relative 1e-10.  The kernel keeps all 1000 generators
+    (1.64 GB of workspace for the job)."""
+    from negftrans_amd import workloads
+    wl = workloads.disorder_c1()
+    NGX, NGY = wl.ngx()
+    vrec = ng.local_potential_real_to_planewave(wl.V_real_eV / O.HARTREE, NGX, NGY)
+    dev = ng.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY)
+    z = wl.mstar * complex(0.015, 0.004) / O.HARTREE
+    got = dev.density_matrix(np.array([z]), np.array([1.0 + 0j]))
+    assert dev.last_status == 0
+    ref = O.diagonal_blocks(oracle_system(wl, np.asarray(vrec), NGX, NGY), z, 0)
+    assert rel_err(got, np.moveaxis(ref, 0, 2)) < RTOL
+
+
 # ------------------------------------------------------------------ a12, a13, a14
 def test_density_small(ng):
     s, V = small_system(N_z=5, seed=21, NK=(2, 1), eta=1e-4)
