@@ -77,47 +77,61 @@ __device__ __forceinline__ cfl from_cd<cfl>(cd v) { return cmk((float)v.x, (floa
 
 #define NEGF_MAXNP 1024   // largest padded block (one workgroup per matrix)
 
-// C = A * B(:, sigB) (sigB == nullptr: identity), Np x Np, every wave computes 32x32
-// tiles with a 4x4 register block per lane.  The epilogue functor receives
-// (row, col, value) for every in-range element.
+typedef double negf_d4 __attribute__((ext_vector_type(4)));
+
+// C = A * B(:, sigB) (sigB == nullptr: identity), Np x Np column-major (Np a multiple of
+// 16), on the FP64 matrix cores.  Every wave computes 32 x 32 output blocks as 2 x 2
+// v_mfma_f64_16x16x4 tiles per real and imaginary part (four real products per complex
+// one).  Operand layout of the 16x16x4 FP64 MFMA: lane i + 16 k supplies A[i][k] and
+// B[k][i]; output element (i, j) sits in lane j + 16 (i % 4), register i / 4.  Operands
+// are read straight from memory in that layout (A: 16 consecutive rows of 4 columns, B:
+// 4 consecutive rows of 16 columns).  Rows and columns past Np are clamped on load and
+// skipped in the epilogue, which receives (row, col, value) for every in-range element.
 template <class Epi>
 __device__ __forceinline__ void gemm_nn(const cd *__restrict__ A, const cd *__restrict__ B, const int *sigB,
-                        const int Np, Epi epi) {
+                                        const int Np, Epi epi) {
     const size_t ld = (size_t)Np;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int tr = lane >> 3, tc = lane & 7;
+    const int li = lane & 15, lk = lane >> 4;
     const int td = (Np + 31) / 32;
     for (int t = w; t < td * td; t += nw) {
-        const int r0 = (t % td) * 32 + tr * 4, c0 = (t / td) * 32 + tc * 4;
-        const cd *bcol[4];
-        int rr[4];
+        const int r0 = (t % td) * 32, c0 = (t / td) * 32;
+        const int ra = min(r0 + li, Np - 1), rb = min(r0 + 16 + li, Np - 1);
+        const int ca = min(c0 + li, Np - 1), cb = min(c0 + 16 + li, Np - 1);
+        const cd *__restrict__ pa = A + ra, *__restrict__ pb = A + rb;
+        const cd *__restrict__ qa = B + (size_t)(sigB ? sigB[ca] : ca) * ld;
+        const cd *__restrict__ qb = B + (size_t)(sigB ? sigB[cb] : cb) * ld;
+        negf_d4 re[2][2], im[2][2];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int c = min(c0 + j, Np - 1);
-            bcol[j] = B + (size_t)(sigB ? sigB[c] : c) * ld;
-            rr[j] = min(r0 + j, Np - 1);
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                re[a][b] = negf_d4{0.0, 0.0, 0.0, 0.0};
+                im[a][b] = negf_d4{0.0, 0.0, 0.0, 0.0};
+            }
+        for (int k0 = 0; k0 < Np; k0 += 4) {
+            const int k = k0 + lk;
+            const cd av[2] = {pa[(size_t)k * ld], pb[(size_t)k * ld]};
+            const cd bv[2] = {qa[k], qb[k]};
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].x, bv[b].x, re[a][b], 0, 0, 0);
+                    re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[a].y, bv[b].y, re[a][b], 0, 0, 0);
+                    im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].x, bv[b].y, im[a][b], 0, 0, 0);
+                    im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].y, bv[b].x, im[a][b], 0, 0, 0);
+                }
         }
-        cd acc[4][4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = cmk(0.0, 0.0);
-        for (int k = 0; k < Np; ++k) {
-            cd a[4], b[4];
+            for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = A[rr[i] + k * ld];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = bcol[j][k];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = cfma(a[i], b[j], acc[i][j]);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (r0 + i < Np && c0 + j < Np) epi(r0 + i, c0 + j, acc[i][j]);
+                for (int q = 0; q < 4; ++q) {
+                    const int r = r0 + 16 * a + 4 * q + lk, c = c0 + 16 * b + li;
+                    if (r < Np && c < Np) epi(r, c, cmk(re[a][b][q], im[a][b][q]));
+                }
     }
 }
 
@@ -539,7 +553,6 @@ __device__ __forceinline__ cd lane_swap(cd v) { return cmk(lane_swap(v.x), lane_
 // ====================================================================================
 #define NEGF_MAXNP4 512
 
-typedef double negf_d4 __attribute__((ext_vector_type(4)));
 typedef float negf_f4 __attribute__((ext_vector_type(4)));
 typedef int negf_i4 __attribute__((ext_vector_type(4)));
 typedef int negf_i2 __attribute__((ext_vector_type(2)));
