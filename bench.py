@@ -50,16 +50,18 @@ def _cpu_worker(args):
     return time.perf_counter() - t0, tau
 
 
-def cpu_baseline(vrec, NGX, NGY, atomic, energies, workers):
+def cpu_baseline(vrec, NGX, NGY, atomic, energies, workers, all_energies=False):
     """The oracle (numpy + scipy's OpenBLAS zgetrf/zgetri/zgemm, the routines the
     reference links) timed on the host: `workers` single-threaded processes, one energy
-    each, round-robin like the reference's MPI ranks; rate = energies / wall."""
+    each (all_energies: every energy, handed out round-robin like the reference's MPI
+    ranks); rate = energies / wall."""
     import multiprocessing as mp
     os.environ["OPENBLAS_NUM_THREADS"] = "1"
     os.environ["OMP_NUM_THREADS"] = "1"
     import dataclasses
     akw = dataclasses.asdict(atomic)
-    jobs = [(vrec, NGX, NGY, akw, float(e)) for e in energies[:workers]]
+    chosen = energies if all_energies else energies[:workers]
+    jobs = [(vrec, NGX, NGY, akw, float(e)) for e in chosen]
     ctx = mp.get_context("spawn")
     t0 = time.perf_counter()
     with ctx.Pool(workers) as pool:
@@ -192,7 +194,27 @@ def main():
                                           f"single-threaded worker, wall {cwall:.1f} s"}
         gpu_tau = dev64.transmission(sample)
         line["cpu_baseline"]["max_abs_dT_vs_gpu"] = float(np.max(np.abs(np.array(taus) - gpu_tau)))
+        line["other_configs"] = {"qpc_c0": qpc_c0_line(np, negf, workloads, workers)}
     print(json.dumps(line), flush=True)
+
+
+def qpc_c0_line(np, negf, workloads, workers):
+    """The reference's own CPU-runnable case (N = 61, N_z = 200, all 400 energies of
+    0-20 meV) on the GPU (FP64, HIP events, best of 3 launches after a warm-up) and on
+    the host oracle (every energy, `workers` single-threaded processes)."""
+    wl = workloads.qpc_c0()
+    NGX, NGY = wl.ngx()
+    vrec = negf.local_potential_real_to_planewave(wl.V_real_eV / negf.HARTREE, NGX, NGY)
+    dev = negf.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY)
+    E = wl.energies_hartree()
+    T = dev.transmission(E)
+    t = min((dev.transmission(E), dev.last_kernel_seconds)[1] for _ in range(3))
+    print(f"# qpc_c0 GPU: {len(E)} energies in {t:.4f} s", file=sys.stderr, flush=True)
+    rate, cwall, taus = cpu_baseline(np.asarray(vrec), NGX, NGY, wl.atomic(), E, workers, all_energies=True)
+    print(f"# qpc_c0 CPU: {len(E)} energies in {cwall:.2f} s", file=sys.stderr, flush=True)
+    return {"N_pw": dev.N, "N_z": dev.N_z, "energies": len(E), "gpu_points_per_s": len(E) / t,
+            "gpu_precision": "fp64", "cpu_points_per_s": rate, "cpu_cores": workers, "cpu_kind": "port",
+            "max_abs_dT": float(np.max(np.abs(np.array(taus) - T)))}
 
 
 if __name__ == "__main__":
