@@ -207,8 +207,12 @@ def qpc_c0_line(np, negf, workloads, workers):
     vrec = negf.local_potential_real_to_planewave(wl.V_real_eV / negf.HARTREE, NGX, NGY)
     dev = negf.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY)
     E = wl.energies_hartree()
-    T = dev.transmission(E)
-    t = min((dev.transmission(E), dev.last_kernel_seconds)[1] for _ in range(3))
+    T = dev.transmission(E)            # warm-up, and the result compared with the CPU
+    launches = []
+    for _ in range(3):
+        dev.transmission(E)
+        launches.append(dev.last_kernel_seconds)
+    t = min(launches)
     print(f"# qpc_c0 GPU: {len(E)} energies in {t:.4f} s", file=sys.stderr, flush=True)
     rate, cwall, taus = cpu_baseline(np.asarray(vrec), NGX, NGY, wl.atomic(), E, workers, all_energies=True)
     print(f"# qpc_c0 CPU: {len(E)} energies in {cwall:.2f} s", file=sys.stderr, flush=True)

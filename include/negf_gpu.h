@@ -212,7 +212,8 @@ int negf_ctx_set_potential(negf_ctx *ctx, const double *vrec);
  * the launches of different sizes run concurrently on forked streams.  The inversion
  * variant of each padded size (matrix-core two-lanes-per-row or VALU one-thread-per-row
  * Gauss-Jordan) is chosen by timing both on the first call; NEGF_MULTI_VARIANT=mfma|valu
- * forces one.  Returns 0, the first zero-pivot slice (> 0), or < 0.
+ * forces one.  Returns 0, the first zero-pivot slice (> 0), or < 0.  The contexts are
+ * not owned: they must outlive the negf_multi.
  */
 typedef struct negf_multi negf_multi;
 int negf_multi_create(negf_multi **out, int ndev, negf_ctx *const *ctxs);

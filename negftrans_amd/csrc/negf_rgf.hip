@@ -2265,7 +2265,11 @@ int tune_group(negf_multi *m, negf_multi::Group &g, const TransArgs &a, const st
             (void)hipStreamSynchronize(m->s0);
             for (void *p : keep) (void)hipFree(p);
         };
-        HIPCHK(hipMemcpyAsync(d_cut, cut.data(), cut.size() * sizeof(SysView), hipMemcpyHostToDevice, m->s0));
+        if (hipMemcpyAsync(d_cut, cut.data(), cut.size() * sizeof(SysView), hipMemcpyHostToDevice, m->s0) !=
+            hipSuccess) {
+            free_all();
+            return fail(-11, "autotune probe upload failed");
+        }
         const int probe_jobs = std::min(g.njobs, m->n_cu);
         std::vector<int> poff(probe_jobs + 1), pjob(probe_jobs);
         for (int j = 0; j <= probe_jobs; ++j) poff[j] = j;
@@ -2276,8 +2280,11 @@ int tune_group(negf_multi *m, negf_multi::Group &g, const TransArgs &a, const st
             free_all();
             return st;
         }
-        HIPCHK(hipMemcpyAsync(d_poff, poff.data(), poff.size() * sizeof(int), hipMemcpyHostToDevice, m->s0));
-        HIPCHK(hipMemcpyAsync(d_pjob, pjob.data(), pjob.size() * sizeof(int), hipMemcpyHostToDevice, m->s0));
+        if (hipMemcpyAsync(d_poff, poff.data(), poff.size() * sizeof(int), hipMemcpyHostToDevice, m->s0) != hipSuccess ||
+            hipMemcpyAsync(d_pjob, pjob.data(), pjob.size() * sizeof(int), hipMemcpyHostToDevice, m->s0) != hipSuccess) {
+            free_all();
+            return fail(-11, "autotune probe upload failed");
+        }
         double best_ms = 1e300;
         for (int v : cand) {
             const size_t per_slot = ((size_t)2 * g.Np * g.Np + (size_t)2 * nLmax * g.Np +

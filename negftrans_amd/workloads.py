@@ -188,6 +188,8 @@ def uneven_c2(seed=20261203, n_devices=48, n_energy=256, sizes=(61, 113, 197, 31
 def fourier_features(V_eV, shape=(16, 16, 8), scale=0.010 * MSTAR_GAAS):
     """The shape[0] x shape[1] x shape[2] lowest Fourier components of V (eV), as the
     band-limited real samples on a grid of that shape, divided by `scale`."""
+    if any(n < s for s, n in zip(shape, V_eV.shape)):
+        raise ValueError(f"potential grid {V_eV.shape} is smaller than the feature grid {shape}")
     F = np.fft.fftn(V_eV)
     idx = [np.r_[0:s // 2, n - s // 2:n] for s, n in zip(shape, V_eV.shape)]
     sub = F[np.ix_(*idx)]
