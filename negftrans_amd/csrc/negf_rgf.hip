@@ -226,6 +226,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission(TransArgs a) {
 struct MixedArgs {
     TransArgs a;      // job tables (a.ws unused)
     int nref;         // refinement iterations
+    double tol;       // stop when the last correction of X_{N_z} is below tol |X_{N_z}|
     int ldb;          // columns of X and Y (max n_R over the jobs, <= NEGF_NRH)
     cfl *hist;        // per slot N_z Np^2: g_m, column-permuted like the Gauss-Jordan output
     int *hsig;        // per slot N_z Np: their column permutations
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
             __syncthreads();
         }
         // 3. iterative refinement with FP64 residuals, until the correction of X_{N_z} on
-        // the open rows is below 1e-9 of |X_{N_z}| there (at most ma.nref iterations;
+        // the open rows is below ma.tol of |X_{N_z}| there (at most ma.nref iterations;
         // a job that does not get there reports status -1)
         bool conv = false;
         for (int it = 0; it < ma.nref && !conv; ++it) {
@@ -486,7 +487,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
             }
             dmx = block_max2(dmx, s.red);
             xmx = block_max2(xmx, s.red);
-            conv = dmx <= 1e-9 * xmx;
+            conv = dmx <= ma.tol * xmx;
         }
         // 4. trace on X_{N_z} = G_{N_z,1}(:, open)
         double part = 0.0;
@@ -989,6 +990,7 @@ struct negf_ctx {
     void *ws = nullptr;
     size_t ws_bytes = 0;
     int mixed_refine = 8;   // refinement iteration limit of NEGF_PREC_MIXED_FP32
+    double mixed_tol = 1e-9; // its stopping tolerance (relative correction of X_{N_z})
     int last_fallbacks = 0; // energies of the last mixed call redone in FP64
 };
 
@@ -1271,6 +1273,7 @@ int negf_ctx_create(negf_ctx **out, int N, int N_z, const int *nx_grid, const in
     c->NGY = NGY;
     c->flags = flags;
     if (const char *r = std::getenv("NEGF_MIXED_REFINE")) c->mixed_refine = std::max(0, std::atoi(r));
+    if (const char *r = std::getenv("NEGF_MIXED_TOL")) c->mixed_tol = std::max(1e-15, std::atof(r));
     c->Lx = Lx;
     c->Ly = Ly;
     c->Lz = Lz;
@@ -1448,6 +1451,7 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
         ma.a.tau = d_tau;
         ma.a.status = d_st;
         ma.nref = c->mixed_refine;
+        ma.tol = c->mixed_tol;
         ma.ldb = nRmax;
         char *base = (char *)c->ws;
         ma.hist = (cfl *)base;
