@@ -81,8 +81,8 @@ typedef double negf_d4 __attribute__((ext_vector_type(4)));
 
 // C = A * B(:, sigB) (sigB == nullptr: identity), Np x Np column-major (Np a multiple of
 // 16), on the FP64 matrix cores.  Every wave computes 32 x 32 output blocks as 2 x 2
-// v_mfma_f64_16x16x4 tiles per real and imaginary part (four real products per complex
-// one).  Operand layout of the 16x16x4 FP64 MFMA: lane i + 16 k supplies A[i][k] and
+// v_mfma_f64_16x16x4 tiles with three real products per complex one (the 3M form:
+// P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); re = P1 - P2, im = P3 - P1 - P2).  Operand layout of the 16x16x4 FP64 MFMA: lane i + 16 k supplies A[i][k] and
 // B[k][i]; output element (i, j) sits in lane j + 16 (i % 4), register i / 4.  Operands
 // are read straight from memory in that layout (A: 16 consecutive rows of 4 columns, B:
 // 4 consecutive rows of 16 columns).  Rows and columns past Np are clamped on load and
@@ -101,13 +101,14 @@ __device__ __forceinline__ void gemm_nn(const cd *__restrict__ A, const cd *__re
         const cd *__restrict__ pa = A + ra, *__restrict__ pb = A + rb;
         const cd *__restrict__ qa = B + (size_t)(sigB ? sigB[ca] : ca) * ld;
         const cd *__restrict__ qb = B + (size_t)(sigB ? sigB[cb] : cb) * ld;
-        negf_d4 re[2][2], im[2][2];
+        negf_d4 p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
-                re[a][b] = negf_d4{0.0, 0.0, 0.0, 0.0};
-                im[a][b] = negf_d4{0.0, 0.0, 0.0, 0.0};
+                p1[a][b] = negf_d4{0.0, 0.0, 0.0, 0.0};
+                p2[a][b] = negf_d4{0.0, 0.0, 0.0, 0.0};
+                p3[a][b] = negf_d4{0.0, 0.0, 0.0, 0.0};
             }
         for (int k0 = 0; k0 < Np; k0 += 4) {
             const int k = k0 + lk;
@@ -117,10 +118,10 @@ __device__ __forceinline__ void gemm_nn(const cd *__restrict__ A, const cd *__re
             for (int a = 0; a < 2; ++a)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].x, bv[b].x, re[a][b], 0, 0, 0);
-                    re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[a].y, bv[b].y, re[a][b], 0, 0, 0);
-                    im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].x, bv[b].y, im[a][b], 0, 0, 0);
-                    im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].y, bv[b].x, im[a][b], 0, 0, 0);
+                    p1[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].x, bv[b].x, p1[a][b], 0, 0, 0);
+                    p2[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].y, bv[b].y, p2[a][b], 0, 0, 0);
+                    p3[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].x + av[a].y, bv[b].x + bv[b].y, p3[a][b],
+                                                                     0, 0, 0);
                 }
         }
 #pragma unroll
@@ -130,7 +131,8 @@ __device__ __forceinline__ void gemm_nn(const cd *__restrict__ A, const cd *__re
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int r = r0 + 16 * a + 4 * q + lk, c = c0 + 16 * b + li;
-                    if (r < Np && c < Np) epi(r, c, cmk(re[a][b][q], im[a][b][q]));
+                    const double re = p1[a][b][q] - p2[a][b][q], im = p3[a][b][q] - p1[a][b][q] - p2[a][b][q];
+                    if (r < Np && c < Np) epi(r, c, cmk(re, im));
                 }
     }
 }
