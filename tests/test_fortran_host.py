@@ -219,3 +219,37 @@ def test_fortran_host_decimation_leads(tmp_path):
     Dref = O.density_real(DM, s.nx, s.ny, N_x, N_y, CASE["LX"], CASE["LY"], CASE["LZ"])
     Dgot = np.loadtxt(tmp_path / "DENSITY_FULL", skiprows=1).ravel().reshape((N_x, N_y, N_z), order="F")
     assert np.linalg.norm(Dgot - Dref) <= 1e-10 * np.linalg.norm(Dref)
+
+
+@pytest.mark.gpu
+def test_fortran_host_qpc_c0(tmp_path):
+    """NEGFtrans's own case at full size (N = 61, N_z = 200, 400 energies of 0-20 meV, m*
+    scaled) through the driver, as a user would run it: INPUT + POTENTIAL in, TRANSMISSION,
+    DENSITY and LDOS out.  TRANSMISSION equals the library's batched call on the same
+    system to 1e-12 (that call is held to the oracle by the parity tests), 0 <= T <= open
+    channels, and OUTPUT reports the phase timers."""
+    from negftrans_amd import negf, workloads
+    wl = workloads.qpc_c0()
+    ms = wl.mstar
+    inp = dict(wl.inp, if_density=True, if_LDOS=True, DUMP_FULL=True, MU=ms * 0.012, TEMPERATURE=10.0,
+               N_circle=40, N_line_per_eV=2000, LDOS_ENERGY_GRID=[0.0, ms * 0.02, 4], LDOS_GRID=[0, -1, 100])
+    write_case(tmp_path, inp, wl.V_real_eV)
+    r = run(tmp_path, timeout=600)
+    out = open(tmp_path / "OUTPUT").read()
+    assert r.returncode == 0, r.stdout + r.stderr + out
+    got = np.loadtxt(tmp_path / "TRANSMISSION_FULL")
+    assert got.shape == (400, 2)
+    NGX, NGY = wl.ngx()
+    vrec = negf.local_potential_real_to_planewave(wl.V_real_eV / negf.HARTREE, NGX, NGY)
+    dev = negf.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY)
+    E = wl.energies_hartree()
+    np.testing.assert_allclose(got[:, 0], E * O.HARTREE, rtol=1e-14, atol=1e-18)
+    ref = dev.transmission(E)
+    np.testing.assert_allclose(got[:, 1], ref, rtol=1e-12, atol=1e-14)
+    a = wl.atomic()
+    nx, ny = wl.basis()
+    kin = O.kinetic(nx, ny, a.LX, a.LY, 0.0, 0.0)
+    n_open = np.array([np.sum(2 * e - kin > 0) for e in E])
+    assert np.all(got[:, 1] >= -1e-12) and np.all(got[:, 1] <= n_open + 1e-6) and got[:, 1].max() > 1.0
+    assert (tmp_path / "DENSITY").exists() and (tmp_path / "LDOS").exists()
+    print([l.strip() for l in out.splitlines() if "Wall time" in l or "|--" in l])
