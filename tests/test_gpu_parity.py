@@ -340,3 +340,23 @@ def test_transmission_c3_block_size(ng):
     n_open = np.array([np.sum(2 * e - kin > 0) for e in E])
     assert np.all(fwd >= -1e-12) and np.all(fwd <= n_open + 1e-6) and n_open.min() >= 2
     np.testing.assert_allclose(rev, fwd, rtol=1e-6, atol=1e-9)
+
+
+def test_empty_and_closed_batches(ng):
+    """Edge cases of the batched calls: no energies, no contour nodes, and energies below
+    every channel (no open channel: T = 0 exactly, as the reference's sum of zeros)."""
+    s, _ = small_system(N_z=5, seed=31)
+    dev = device_for(ng, s)
+    assert dev.transmission(np.array([])).shape == (0,)
+    dm = dev.density_matrix(np.array([], dtype=np.complex128), np.array([], dtype=np.complex128))
+    assert np.all(dm == 0)
+    assert dev.ldos_matrix(np.array([])).shape[-1] == 0
+    T = dev.transmission(np.array([-0.3, -0.01, 0.0]))
+    assert np.all(T == 0.0)
+    # an uneven batch with a device that has no energies
+    s2, _ = small_system(N_z=7, seed=32)
+    d2 = device_for(ng, s2)
+    E2 = np.linspace(0.1, 0.9, 5)
+    got = ng.Multi([dev, d2]).transmission([np.array([]), E2])
+    assert got[0].shape == (0,)
+    np.testing.assert_allclose(got[1], d2.transmission(E2), rtol=1e-12)
