@@ -11,7 +11,7 @@ module negf
     private
     public Equilibrium_Density, Transmission_Coefficient, Local_Density_Of_State
     public Transmission_batch, Density_batch, LDOS_batch, contour_node, negf_release, negf_precision
-    public negf_contour_rule, Bias_density_batch, negf_lead
+    public negf_contour_rule, Bias_density_batch, negf_lead, LDOS_chunk
 
     type(c_ptr), save :: ctx = c_null_ptr
     complex*16, allocatable, save :: key_V(:, :, :)
@@ -283,6 +283,32 @@ contains
                         "negf_neq_density_batch")
         seconds = negf_last_kernel_seconds(ctx)
     end subroutine Bias_density_batch
+
+    ! LDOS_Matrix for energy indices ie0 .. ie0 + nc - 1 (all k points) in one launch;
+    ! LDOS_Matrix(:, :, (ic - 1) N_z + 1 : ic N_z) holds energy ie0 + ic - 1.
+    subroutine LDOS_chunk(ie0, nc, N_E, V_reciprocal_all, nx_grid, ny_grid, atomic, kpoint, LDOS_Matrix, seconds)
+        integer, intent(in) :: ie0, nc, N_E, nx_grid(:), ny_grid(:)
+        complex*16, intent(in) :: V_reciprocal_all(:, :, :)
+        type(t_parameters), intent(in) :: atomic
+        type(t_kpointmesh), intent(in) :: kpoint(:)
+        complex*16, intent(out) :: LDOS_Matrix(:, :, :)
+        real*8, intent(out) :: seconds
+        real(c_double), allocatable :: E(:), kx(:), ky(:), kw(:)
+        integer :: ic
+        call bind_system(V_reciprocal_all, nx_grid, ny_grid, atomic)
+        allocate(E(nc))
+        do ic = 1, nc
+            ! src/negf.f90:203-204, as LDOS_batch
+            E(ic) = atomic%LDOS_ENERGY_GRID(1) + &
+                    (atomic%LDOS_ENERGY_GRID(2) - atomic%LDOS_ENERGY_GRID(1)) * (ie0 + ic - 2) / N_E
+        end do
+        allocate(kx(size(kpoint)), ky(size(kpoint)), kw(size(kpoint)))
+        kx = kpoint(:)%kx
+        ky = kpoint(:)%ky
+        kw = kpoint(:)%weight
+        call negf_check(negf_ldos_batch(ctx, E, nc, kx, ky, kw, size(kx), LDOS_Matrix), "negf_ldos_batch")
+        seconds = negf_last_kernel_seconds(ctx)
+    end subroutine LDOS_chunk
 
     ! LDOS_Matrix for one energy index (all k points in one launch).
     subroutine LDOS_batch(i_energy, N_E, V_reciprocal_all, nx_grid, ny_grid, atomic, kpoint, LDOS_Matrix, seconds)

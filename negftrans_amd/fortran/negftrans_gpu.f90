@@ -48,6 +48,7 @@ program negftrans_gpu
     complex(c_double_complex), allocatable :: realspace(:, :, :)
     integer, allocatable :: nx_grid(:), ny_grid(:)
     integer(c_int), allocatable :: cnx(:), cny(:)
+    integer :: nchunk, nc, ie0, ic
     integer :: N_x, N_y, N_z, N, N_E, N_line, N_window, i, j, k, ii, jj, kk, ie, io, bound
     real*8 :: rescale_factor, gpu_s, t_total0, t_total1, t_rtop, t_dens, t_ptor, t_trans, t_ldos, t_ldos_ptor
     real*8 :: t0, t1, gpu_s2
@@ -250,32 +251,40 @@ program negftrans_gpu
 
     if (if_LDOS) then
         N_E = nint(LDOS_ENERGY_GRID(3))
-        allocate(LDOS_Matrix(N, N, N_z), realspace(N_x, N_y, N_z))
+        ! energies per GPU call: as many as fit 2 GB of host blocks and real-space slices
+        nchunk = max(1, min(N_E, int(2.0D9 / (16.D0 * (dble(N) * N * N_z + dble(N_x) * N_y * N_z)))))
+        allocate(LDOS_Matrix(N, N, N_z * nchunk), realspace(N_x, N_y, N_z * nchunk))
         allocate(LDOS(LDOS_size(LDOS_GRID(1), N_x), LDOS_size(LDOS_GRID(2), N_y), &
                       LDOS_size(LDOS_GRID(3), N_z), N_E))
         LDOS = 0.D0
         write(16, *) "LDOS calculation start..."
         write(16, *) "Number of energy grid points:", N_E
-        do ie = 1, N_E
+        do ie0 = 1, N_E, nchunk
+            nc = min(nchunk, N_E - ie0 + 1)
             call wall(t0)
-            call LDOS_batch(ie, N_E, V_reciprocal_all, nx_grid, ny_grid, atomic, kpoint, LDOS_Matrix, gpu_s)
+            call LDOS_chunk(ie0, nc, N_E, V_reciprocal_all, nx_grid, ny_grid, atomic, kpoint, LDOS_Matrix, gpu_s)
             call wall(t1)
             t_ldos = t_ldos + (t1 - t0)
-            call negf_check(negf_pw_to_real(N, cnx, cny, N_z, LDOS_Matrix, N_x, N_y, realspace), "negf_pw_to_real")
-            do k = 1, N_z
-                do j = 1, N_y
-                    do i = 1, N_x
-                        rescale_factor = dble(N_z) / (LX * LY * LZ) / hartree
-                        call LDOS_grid_converter(i, LDOS_GRID(1), LX / N_x, ii, rescale_factor)
-                        call LDOS_grid_converter(j, LDOS_GRID(2), LY / N_y, jj, rescale_factor)
-                        call LDOS_grid_converter(k, LDOS_GRID(3), LZ / N_z, kk, rescale_factor)
-                        LDOS(ii, jj, kk, ie) = LDOS(ii, jj, kk, ie) + aimag(realspace(i, j, k)) * rescale_factor
+            call negf_check(negf_pw_to_real(N, cnx, cny, N_z * nc, LDOS_Matrix, N_x, N_y, realspace), &
+                            "negf_pw_to_real")
+            do ic = 1, nc
+                ie = ie0 + ic - 1
+                do k = 1, N_z
+                    do j = 1, N_y
+                        do i = 1, N_x
+                            rescale_factor = dble(N_z) / (LX * LY * LZ) / hartree
+                            call LDOS_grid_converter(i, LDOS_GRID(1), LX / N_x, ii, rescale_factor)
+                            call LDOS_grid_converter(j, LDOS_GRID(2), LY / N_y, jj, rescale_factor)
+                            call LDOS_grid_converter(k, LDOS_GRID(3), LZ / N_z, kk, rescale_factor)
+                            LDOS(ii, jj, kk, ie) = LDOS(ii, jj, kk, ie) + &
+                                                   aimag(realspace(i, j, (ic - 1) * N_z + k)) * rescale_factor
+                        end do
                     end do
                 end do
+                write(16, '(A2, I6, A2, I6)') "=>", ie, " /", N_E
             end do
             call wall(t0)
             t_ldos_ptor = t_ldos_ptor + (t0 - t1)
-            write(16, '(A2, I6, A2, I6)') "=>", ie, " /", N_E
         end do
         write(16, *) "LDOS calculation DONE"
         write(16, *) "----------"

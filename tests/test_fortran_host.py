@@ -232,7 +232,7 @@ def test_fortran_host_qpc_c0(tmp_path):
     wl = workloads.qpc_c0()
     ms = wl.mstar
     inp = dict(wl.inp, if_density=True, if_LDOS=True, DUMP_FULL=True, MU=ms * 0.012, TEMPERATURE=10.0,
-               N_circle=40, N_line_per_eV=2000, LDOS_ENERGY_GRID=[0.0, ms * 0.02, 4], LDOS_GRID=[0, -1, 100])
+               N_circle=40, N_line_per_eV=2000, LDOS_ENERGY_GRID=[0.0, ms * 0.02, 40], LDOS_GRID=[0, -1, 100])
     write_case(tmp_path, inp, wl.V_real_eV)
     r = run(tmp_path, timeout=600)
     out = open(tmp_path / "OUTPUT").read()
