@@ -529,6 +529,7 @@ struct DiagArgs {
     cd *ws;               // per slot (N_z + 2) Np^2
     cd *out;              // N*N*N_z per target, column-major ld N
     int *status;
+    int phase, job_base;  // k_diagonal_split only
 };
 
 template <int MAXT, int PB, bool TWO, bool MULTI = false>
@@ -605,6 +606,105 @@ __global__ __launch_bounds__(MAXT) void k_diagonal(DiagArgs a) {
         if (threadIdx.x == 0) a.status[job] = bad;
         __syncthreads();
     }
+}
+
+// D mode cut in two (same blocks, same operation count).  With the forward generators
+// from the left lead, f_1 = D_1^{-1}, f_k = (D_k - f_{k-1})^{-1}, beside the backward ones
+// g_k above, the diagonal blocks at a cut m = N_z / 2 are
+//   G_mm = (D_m - f_{m-1} - g_{m+1})^{-1},   G_{m+1,m+1} = (D_{m+1} - f_m - g_{m+2})^{-1},
+// and the recursion runs outwards from the cut on either side:
+//   G_kk = f_k (G_{k+1,k+1} f_k + I)  (k < m),   G_kk = g_k (G_{k-1,k-1} g_k + I)  (k > m + 1).
+// Two workgroups per job, one per side.  Phase 0 stores the generators of its side
+// (f_1..f_m or g_{N_z}..g_{m+1}: N_z / 2 inversions each); phase 1, a second launch, makes
+// the cut block of its side from both sides' generators and runs the products.  A job's
+// critical path is half of k_diagonal's, and a batch of fewer jobs than CUs fills twice
+// as many.  Per job the workspace holds G[k - 1] = f_k (k <= m) or g_k (k > m), then per
+// side X, Y, W (Np^2 each) and the Gauss-Jordan scratch.  Needs N_z >= 2.
+template <int MAXT, int PB, bool TWO>
+__global__ __launch_bounds__(MAXT) void k_diagonal_split(DiagArgs a) {
+    __shared__ GjeSh<PB, TWO> s;
+    const SysView &sv = a.sv;
+    const int Np = sv.Np, N = sv.N, Nz = sv.Nz, m = Nz / 2;
+    const size_t NN = (size_t)Np * Np;
+    const size_t per_job = (size_t)(Nz + 6) * NN + 4 * (size_t)PB * Np;
+    const int jl = blockIdx.x >> 1, side = blockIdx.x & 1;
+    const int job = a.job_base + jl;
+    cd *G = a.ws + (size_t)jl * per_job;
+    cd *X = G + (size_t)(Nz + 3 * side) * NN, *Y = X + NN, *W = Y + NN;
+    cd *scratch = G + (size_t)(Nz + 6) * NN + (size_t)side * 2 * PB * Np;
+    JobView jv;
+    jv.z = a.z[job];
+    jv.kinh = a.kinh + (size_t)a.kidx[job] * Np;
+    jv.sigL = a.sigL + (size_t)job * Np;
+    jv.sigR = a.sigR + (size_t)job * Np;
+    int *status = a.status + 2 * (size_t)job + side;
+    if (a.phase == 0) {
+        int bad = 0;
+        if (side == 0) {
+            for (int k = 1; k <= m; ++k) {
+                build_block(W, sv, jv, k - 1, k == 1 ? nullptr : G + (size_t)(k - 2) * NN, nullptr);
+                __syncthreads();
+                if (gje_any<PB, TWO>(W, Np, s, scratch) && !bad) bad = k;
+                copy_cols(G + (size_t)(k - 1) * NN, W, s.sigma, Np);
+                __syncthreads();
+            }
+        } else {
+            for (int k = Nz; k > m; --k) {
+                build_block(W, sv, jv, k - 1, k == Nz ? nullptr : G + (size_t)k * NN, nullptr);
+                __syncthreads();
+                if (gje_any<PB, TWO>(W, Np, s, scratch) && !bad) bad = k;
+                copy_cols(G + (size_t)(k - 1) * NN, W, s.sigma, Np);
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x == 0) *status = bad;
+        return;
+    }
+    const cd wgt = a.wgt[job];
+    cd *out = a.out + (size_t)a.target[job] * N * N * Nz;
+    // the cut block of this side: slice c (1-based), neighbours f_{c-1} and g_{c+1}
+    const int c = side == 0 ? m : m + 1;
+    const cd *fl = c >= 2 ? G + (size_t)(c - 2) * NN : nullptr;
+    const cd *gr = c + 1 <= Nz ? G + (size_t)c * NN : nullptr;
+    for (size_t e = threadIdx.x; e < NN; e += blockDim.x)
+        Y[e] = fl && gr ? cadd(fl[e], gr[e]) : (fl ? fl[e] : gr[e]);
+    __syncthreads();
+    build_block(W, sv, jv, c - 1, Y, nullptr);
+    __syncthreads();
+    const int zp = gje_any<PB, TWO>(W, Np, s, scratch);
+    copy_cols(X, W, s.sigma, Np);
+    __syncthreads();
+    auto accumulate = [&](cd *ok, int r, int col, cd v) {
+        const cd t = cmul(wgt, v);
+        double *o = reinterpret_cast<double *>(ok + r + (size_t)col * N);
+        atomicAdd(o, t.x);
+        atomicAdd(o + 1, t.y);
+    };
+    {
+        cd *o0 = out + (size_t)(c - 1) * N * N;
+        for (int col = 0; col < N; ++col)
+            for (int r = threadIdx.x; r < N; r += blockDim.x) accumulate(o0, r, col, X[r + (size_t)col * Np]);
+    }
+    __syncthreads();
+    // outwards from the cut: k = c - 1 .. 1 on the left, c + 1 .. N_z on the right
+    const int nsteps = side == 0 ? c - 1 : Nz - c;
+    for (int step = 0; step < 2 * nsteps; ++step) {
+        const int k = side == 0 ? c - 1 - (step >> 1) : c + 1 + (step >> 1);   // 1-based
+        const bool second = step & 1;
+        const cd *gk = G + (size_t)(k - 1) * NN;
+        cd *ok = out + (size_t)(k - 1) * N * N;
+        gemm_nn(second ? gk : X, second ? Y : gk, nullptr, Np, [&](int r, int col, cd v) {
+            if (!second) {
+                if (r == col) v.x += 1.0;
+                Y[r + (size_t)col * Np] = v;
+            } else {
+                X[r + (size_t)col * Np] = v;
+                if (r < N && col < N) accumulate(ok, r, col, v);
+            }
+        });
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && zp && *status == 0) *status = c;
 }
 
 // Bias-window spectral blocks for the non-equilibrium density (extension, no reference
@@ -992,6 +1092,7 @@ struct negf_ctx {
     int mixed_refine = 8;   // refinement iteration limit of NEGF_PREC_MIXED_FP32
     double mixed_tol = 1e-9; // its stopping tolerance (relative correction of X_{N_z})
     int last_fallbacks = 0; // energies of the last mixed call redone in FP64
+    bool diag_one_sided = false; // D mode: never the two-sided form (NEGF_DIAG_ONE_SIDED=1)
 };
 
 namespace {
@@ -1274,6 +1375,7 @@ int negf_ctx_create(negf_ctx **out, int N, int N_z, const int *nx_grid, const in
     c->flags = flags;
     if (const char *r = std::getenv("NEGF_MIXED_REFINE")) c->mixed_refine = std::max(0, std::atoi(r));
     if (const char *r = std::getenv("NEGF_MIXED_TOL")) c->mixed_tol = std::max(1e-15, std::atof(r));
+    if (const char *r = std::getenv("NEGF_DIAG_ONE_SIDED")) c->diag_one_sided = std::atoi(r) != 0;
     c->Lx = Lx;
     c->Ly = Ly;
     c->Lz = Lz;
@@ -1568,7 +1670,7 @@ static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &e
         (st = tmp.upload(&d_sl, t.sigL.data(), njp, c->stream)) ||
         (st = tmp.upload(&d_sr, t.sigR.data(), njp, c->stream)) ||
         (st = tmp.upload(&d_w, hw.data(), nj, c->stream)) ||
-        (st = tmp.upload(&d_t, target.data(), nj, c->stream)) || (st = tmp.alloc(&d_st, nj)))
+        (st = tmp.upload(&d_t, target.data(), nj, c->stream)) || (st = tmp.alloc(&d_st, 2 * (size_t)nj)))
         return st;
     if (accumulate) {
         if ((st = tmp.upload(&d_out, reinterpret_cast<const cd *>(host_out), nout, c->stream))) return st;
@@ -1578,13 +1680,6 @@ static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &e
     }
     const int threads = threads_for_shape(c->Np);
     const Shape shp = shape_for(c->Np);
-    const size_t per_slot =
-        ((size_t)(c->Nz + 2) * c->Np * c->Np + (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
-    const void *kfn = shp == SHAPE_640 ? (const void *)k_diagonal<640, 32, true>
-                    : shp == SHAPE_1024_TWO ? (const void *)k_diagonal<1024, 32, true>
-                                            : (const void *)k_diagonal<1024, 16, false>;
-    const int slots = slots_for(c, kfn, threads, nj, per_slot);
-    if ((st = ensure_ws(c, per_slot * slots))) return st;
     a.z = d_z;
     a.kidx = d_k;
     a.kinh = d_kin;
@@ -1592,9 +1687,53 @@ static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &e
     a.sigR = d_sr;
     a.wgt = d_w;
     a.target = d_t;
-    a.ws = (cd *)c->ws;
     a.out = d_out;
     a.status = d_st;
+    // Few jobs (at most half the workgroups the GPU holds at once): the two-sided form,
+    // two workgroups per job
+    const void *kfs = shp == SHAPE_640 ? (const void *)k_diagonal_split<640, 32, true>
+                    : shp == SHAPE_1024_TWO ? (const void *)k_diagonal_split<1024, 32, true>
+                                            : (const void *)k_diagonal_split<1024, 16, false>;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfs, threads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    if (c->Nz >= 2 && nj >= 1 && 2L * nj <= (long)c->n_cu * per_cu && !c->diag_one_sided) {
+        const size_t per_job = ((size_t)(c->Nz + 6) * c->Np * c->Np + (size_t)4 * panel_width(c->Np) * c->Np) * sizeof(cd);
+        const int P = slots_for(c, kfs, threads, nj, per_job);
+        if ((st = ensure_ws(c, per_job * P))) return st;
+        a.ws = (cd *)c->ws;
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        for (int base = 0; base < nj; base += P) {
+            a.job_base = base;
+            const int grid = 2 * std::min(P, nj - base);
+            for (int ph = 0; ph < 2; ++ph) {
+                a.phase = ph;
+                if (shp == SHAPE_640) hipLaunchKernelGGL((k_diagonal_split<640, 32, true>), dim3(grid), dim3(threads), 0, c->stream, a);
+                else if (shp == SHAPE_1024_TWO) hipLaunchKernelGGL((k_diagonal_split<1024, 32, true>), dim3(grid), dim3(threads), 0, c->stream, a);
+                else hipLaunchKernelGGL((k_diagonal_split<1024, 16, false>), dim3(grid), dim3(threads), 0, c->stream, a);
+                HIPCHK(hipGetLastError());
+            }
+        }
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        std::vector<int> h_st(2 * (size_t)nj);
+        HIPCHK(hipMemcpyAsync(host_out, d_out, nout * sizeof(cd), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(h_st.data(), d_st, 2 * (size_t)nj * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->last_kernel_s = ms * 1e-3;
+        for (int j = 0; j < nj; ++j)
+            if (h_st[2 * j] || h_st[2 * j + 1]) return h_st[2 * j] ? h_st[2 * j] : h_st[2 * j + 1];
+        return 0;
+    }
+    const size_t per_slot =
+        ((size_t)(c->Nz + 2) * c->Np * c->Np + (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
+    const void *kfn = shp == SHAPE_640 ? (const void *)k_diagonal<640, 32, true>
+                    : shp == SHAPE_1024_TWO ? (const void *)k_diagonal<1024, 32, true>
+                                            : (const void *)k_diagonal<1024, 16, false>;
+    const int slots = slots_for(c, kfn, threads, nj, per_slot);
+    if ((st = ensure_ws(c, per_slot * slots))) return st;
+    a.ws = (cd *)c->ws;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     if (shp == SHAPE_640) hipLaunchKernelGGL((k_diagonal<640, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
     else if (shp == SHAPE_1024_TWO) hipLaunchKernelGGL((k_diagonal<1024, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);

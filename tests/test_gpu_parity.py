@@ -253,6 +253,29 @@ def test_density_small(ng):
     assert rel_err(got, np.moveaxis(ref, 0, 2)) < RTOL
 
 
+@pytest.mark.parametrize("N_z", [2, 3, 7])
+def test_diagonal_two_sided_and_one_sided(ng, N_z, monkeypatch):
+    """D mode: a batch of few jobs runs two workgroups per job (generators from both
+    leads, diagonal blocks outwards from the cut at N_z / 2); NEGF_DIAG_ONE_SIDED=1 keeps
+    the one-workgroup backward-generator form.  Both against the oracle's tri_solver 'D'
+    restatement, relative 1e-10, down to N_z = 2 (one slice per side; contexts need
+    N_z >= 2)."""
+    s, _ = small_system(N_z=N_z, seed=31, eta=1e-4)
+    zs = np.array([0.3 + 0.05j, 0.9 + 0.002j, 1.4 + 0.3j])
+    dev = device_for(ng, s)
+    two = dev.density_matrix(zs, np.ones(3, dtype=complex))
+    assert dev.last_status == 0
+    monkeypatch.setenv("NEGF_DIAG_ONE_SIDED", "1")
+    dev1 = device_for(ng, s)
+    monkeypatch.delenv("NEGF_DIAG_ONE_SIDED")
+    one = dev1.density_matrix(zs, np.ones(3, dtype=complex))
+    assert dev1.last_status == 0
+    ref = sum(O.diagonal_blocks(s, z, 0) for z in zs)
+    assert rel_err(two, np.moveaxis(ref, 0, 2)) < RTOL
+    assert rel_err(one, np.moveaxis(ref, 0, 2)) < RTOL
+    assert rel_err(two, one) < RTOL
+
+
 def test_ldos_small(ng):
     s, V = small_system(N_z=6, seed=22, NK=(1, 2), eta=1e-3)
     E = np.array([0.2, 0.7])
