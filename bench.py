@@ -195,6 +195,15 @@ def main():
         gpu_tau = dev64.transmission(sample)
         line["cpu_baseline"]["max_abs_dT_vs_gpu"] = float(np.max(np.abs(np.array(taus) - gpu_tau)))
         line["other_configs"] = {"qpc_c0": qpc_c0_line(np, negf, workloads, workers)}
+    if dev64 is not None:
+        # the same device's equilibrium density: 64 Gauss-Legendre arc nodes, one call
+        from negftrans_amd import density_c1
+        dev64.close()
+        d = density_c1.measure(64, 1, log=lambda m: print(m, file=sys.stderr, flush=True))
+        d["flop"] = 64 * 8.0 * N_PW ** 3 * (3 * N_Z - 2)
+        d["achieved_tflops"] = d["flop"] / d["device_s"] / 1e12
+        d["frac_of_fp64_peak"] = d["achieved_tflops"] / PEAK_FP64_TFLOPS
+        line.setdefault("other_configs", {})["density_c1"] = d
     print(json.dumps(line), flush=True)
 
 

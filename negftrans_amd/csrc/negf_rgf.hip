@@ -728,6 +728,7 @@ struct NeqArgs {
     cd *ws;                 // per slot (N_z + 1) Np^2 + 2 nLmax Np + 2 PB Np
     cd *out;                // N*N*N_z, column-major ld N
     int *status;
+    int phase, job_base;    // k_neq_split only
 };
 
 template <int MAXT, int PB, bool TWO>
@@ -802,6 +803,152 @@ __global__ __launch_bounds__(MAXT) void k_neq(NeqArgs a) {
         if (threadIdx.x == 0) a.status[job] = bad;
         __syncthreads();
     }
+}
+
+// The same bias-window blocks cut in two, as k_diagonal_split.  The left workgroup keeps
+// the forward generators f_1..f_m (m = N_z / 2) and the first column of each left part,
+//   L_1 = f_1(:, open),  L_k = -f_k L_{k-1}       (L_k = [(A restricted to 1..k)^{-1}]_{k,1}),
+// the right one the backward generators g_{N_z}..g_{m+1}.  In the second launch both invert
+// the cut block G_mm = (D_m - f_{m-1} - g_{m+1})^{-1}, take G_{m,1} = -G_mm L_{m-1}
+// (G_11 itself when m = 1), and run outwards:
+//   G_{i,1} = L_i - f_i G_{i+1,1}  (i < m; Dyson at the cut between i and i + 1),
+//   G_{k,1} = -g_k G_{k-1,1}       (k > m, as in k_neq),
+// the left one accumulating slices 1..m, the right one m+1..N_z.  Per job the workspace
+// holds G[k - 1] = f_k or g_k, L_1..L_{m-1} (ldV x Np each), then per side W (Np^2),
+// two column blocks and the Gauss-Jordan scratch.
+template <int MAXT, int PB, bool TWO>
+__global__ __launch_bounds__(MAXT) void k_neq_split(NeqArgs a) {
+    __shared__ GjeSh<PB, TWO> s;
+    const SysView &sv = a.sv;
+    const int Np = sv.Np, N = sv.N, Nz = sv.Nz, ldV = a.nLmax, m = Nz / 2;
+    const size_t NN = (size_t)Np * Np, VV = (size_t)ldV * Np;
+    const size_t per_side = NN + 2 * VV + 2 * (size_t)PB * Np;
+    const size_t per_job = (size_t)Nz * NN + (size_t)m * VV + 2 * per_side;
+    const int jl = blockIdx.x >> 1, side = blockIdx.x & 1;
+    const int job = a.job_base + jl;
+    cd *G = a.ws + (size_t)jl * per_job;
+    cd *Lst = G + (size_t)Nz * NN;
+    cd *W = Lst + (size_t)m * VV + side * per_side;
+    cd *V0 = W + NN, *V1 = V0 + VV;
+    cd *scratch = V1 + VV;
+    int *status = a.status + 2 * (size_t)job + side;
+    const int nL = a.nL[job], off = a.open_off[job];
+    if (nL == 0) {                                // no open channel of the left lead
+        if (threadIdx.x == 0) *status = 0;
+        return;
+    }
+    JobView jv;
+    jv.z = a.z[job];
+    jv.kinh = a.kinh + (size_t)a.kidx[job] * Np;
+    jv.sigL = a.sigL + (size_t)job * Np;
+    jv.sigR = a.sigR + (size_t)job * Np;
+    // Vo = add - M(:, sig) Vi on the nL open columns (add == nullptr: -M(:, sig) Vi)
+    auto matvec = [&](cd *Vo, const cd *M, const int *sig, const cd *Vi, const cd *add) {
+        for (int e = threadIdx.x; e < nL * Np; e += blockDim.x) {
+            const int q = e / Np, r = e % Np;
+            const cd *vq = Vi + (size_t)q * Np;
+            cd acc = cmk(0.0, 0.0);
+            for (int c = 0; c < Np; ++c) acc = cfma(M[r + (size_t)(sig ? sig[c] : c) * Np], vq[c], acc);
+            Vo[e] = add ? csub(add[e], acc) : cmk(-acc.x, -acc.y);
+        }
+    };
+    if (a.phase == 0) {
+        int bad = 0;
+        if (side == 0) {
+            for (int k = 1; k <= m; ++k) {
+                build_block(W, sv, jv, k - 1, k == 1 ? nullptr : G + (size_t)(k - 2) * NN, nullptr);
+                __syncthreads();
+                if (gje_any<PB, TWO>(W, Np, s, scratch) && !bad) bad = k;
+                copy_cols(G + (size_t)(k - 1) * NN, W, s.sigma, Np);
+                __syncthreads();
+                if (k < m) {
+                    cd *Lk = Lst + (size_t)(k - 1) * VV;
+                    if (k == 1) {
+                        for (int e = threadIdx.x; e < nL * Np; e += blockDim.x) {
+                            const int q = e / Np, r = e % Np;
+                            Lk[e] = G[r + (size_t)a.Lcol[off + q] * Np];
+                        }
+                    } else {
+                        matvec(Lk, G + (size_t)(k - 1) * NN, nullptr, Lst + (size_t)(k - 2) * VV, nullptr);
+                    }
+                    __syncthreads();
+                }
+            }
+        } else {
+            for (int k = Nz; k > m; --k) {
+                build_block(W, sv, jv, k - 1, k == Nz ? nullptr : G + (size_t)k * NN, nullptr);
+                __syncthreads();
+                if (gje_any<PB, TWO>(W, Np, s, scratch) && !bad) bad = k;
+                copy_cols(G + (size_t)(k - 1) * NN, W, s.sigma, Np);
+                __syncthreads();
+            }
+        }
+        if (threadIdx.x == 0) *status = bad;
+        return;
+    }
+    const cd wgt = a.wgt[job];
+    auto accumulate = [&](int k, const cd *V) {
+        cd *ok = a.out + (size_t)k * N * N;
+        for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+            const int r = e % N, c = e / N;
+            cd v = cmk(0.0, 0.0);
+            for (int q = 0; q < nL; ++q) {
+                const cd x = V[(size_t)q * Np + r], y = V[(size_t)q * Np + c];
+                const double gq = a.gL[off + q];
+                v.x += gq * (x.x * y.x + x.y * y.y);       // x conj(y)
+                v.y += gq * (x.y * y.x - x.x * y.y);
+            }
+            const cd t = cmul(wgt, v);
+            double *o = reinterpret_cast<double *>(ok + e);
+            atomicAdd(o, t.x);
+            atomicAdd(o + 1, t.y);
+        }
+    };
+    // cut block: W = D_m - g_{m+1} - f_{m-1} on the physical part, inverted in place
+    build_block(W, sv, jv, m - 1, G + (size_t)m * NN, nullptr);
+    __syncthreads();
+    if (m >= 2) {
+        const cd *fl = G + (size_t)(m - 2) * NN;
+        for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+            const size_t o = (size_t)(e % N) + (size_t)(e / N) * Np;
+            W[o] = csub(W[o], fl[o]);
+        }
+    }
+    __syncthreads();
+    const int zp = gje_any<PB, TWO>(W, Np, s, scratch);
+    if (m >= 2) {
+        matvec(V0, W, s.sigma, Lst + (size_t)(m - 2) * VV, nullptr);
+    } else {
+        for (int e = threadIdx.x; e < nL * Np; e += blockDim.x) {
+            const int q = e / Np, r = e % Np;
+            V0[e] = W[r + (size_t)s.sigma[a.Lcol[off + q]] * Np];
+        }
+    }
+    __syncthreads();
+    if (side == 0) {
+        accumulate(m - 1, V0);
+        __syncthreads();
+        for (int i = m - 1; i >= 1; --i) {
+            matvec(V1, G + (size_t)(i - 1) * NN, nullptr, V0, Lst + (size_t)(i - 1) * VV);
+            __syncthreads();
+            accumulate(i - 1, V1);
+            __syncthreads();
+            cd *tt = V0;
+            V0 = V1;
+            V1 = tt;
+        }
+    } else {
+        for (int k = m + 1; k <= Nz; ++k) {
+            matvec(V1, G + (size_t)(k - 1) * NN, nullptr, V0, nullptr);
+            __syncthreads();
+            accumulate(k - 1, V1);
+            __syncthreads();
+            cd *tt = V0;
+            V0 = V1;
+            V1 = tt;
+        }
+    }
+    if (threadIdx.x == 0 && zp && *status == 0) *status = m;
 }
 
 // Batched primitives (drop-ins for inverse / matrix_product, src/math_kernel.f90:8-42,
@@ -1092,7 +1239,7 @@ struct negf_ctx {
     int mixed_refine = 8;   // refinement iteration limit of NEGF_PREC_MIXED_FP32
     double mixed_tol = 1e-9; // its stopping tolerance (relative correction of X_{N_z})
     int last_fallbacks = 0; // energies of the last mixed call redone in FP64
-    bool diag_one_sided = false; // D mode: never the two-sided form (NEGF_DIAG_ONE_SIDED=1)
+    bool diag_one_sided = false; // D mode and bias window: never the two-sided forms (NEGF_DIAG_ONE_SIDED=1)
 };
 
 namespace {
@@ -1955,7 +2102,7 @@ int negf_neq_density_batch(negf_ctx *c, const double *E, const double *w, int nE
         gLv.push_back(0.0);
     }
     Temps tmp;
-    NeqArgs a;
+    NeqArgs a{};
     a.sv = sysview(c);
     a.njobs = nj;
     cd *d_z, *d_sl, *d_sr, *d_w, *d_out;
@@ -1974,17 +2121,10 @@ int negf_neq_density_batch(negf_ctx *c, const double *E, const double *w, int nE
         (st = tmp.upload(&d_Lc, Lcol.data(), Lcol.size(), c->stream)) ||
         (st = tmp.upload(&d_gl, gLv.data(), gLv.size(), c->stream)) ||
         (st = tmp.upload(&d_out, reinterpret_cast<const cd *>(density_matrix), nout, c->stream)) ||
-        (st = tmp.alloc(&d_st, nj)))
+        (st = tmp.alloc(&d_st, 2 * (size_t)nj)))
         return st;
     const int threads = threads_for_shape(c->Np);
     const Shape shp = shape_for(c->Np);
-    const size_t per_slot = ((size_t)(c->Nz + 1) * c->Np * c->Np + (size_t)2 * nLmax * c->Np +
-                             (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
-    const void *kfn = shp == SHAPE_640 ? (const void *)k_neq<640, 32, true>
-                    : shp == SHAPE_1024_TWO ? (const void *)k_neq<1024, 32, true>
-                                            : (const void *)k_neq<1024, 16, false>;
-    const int slots = slots_for(c, kfn, threads, nj, per_slot);
-    if ((st = ensure_ws(c, per_slot * slots))) return st;
     a.z = d_z;
     a.kidx = d_k;
     a.kinh = d_kin;
@@ -1996,9 +2136,54 @@ int negf_neq_density_batch(negf_ctx *c, const double *E, const double *w, int nE
     a.Lcol = d_Lc;
     a.gL = d_gl;
     a.nLmax = nLmax;
-    a.ws = (cd *)c->ws;
     a.out = d_out;
     a.status = d_st;
+    // few jobs: two workgroups per job (k_neq_split), as in D mode
+    const void *kfs = shp == SHAPE_640 ? (const void *)k_neq_split<640, 32, true>
+                    : shp == SHAPE_1024_TWO ? (const void *)k_neq_split<1024, 32, true>
+                                            : (const void *)k_neq_split<1024, 16, false>;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfs, threads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    if (c->Nz >= 2 && 2L * nj <= (long)c->n_cu * per_cu && !c->diag_one_sided) {
+        const size_t NN = (size_t)c->Np * c->Np, VV = (size_t)nLmax * c->Np;
+        const size_t per_job = ((size_t)c->Nz * NN + (size_t)(c->Nz / 2) * VV +
+                                2 * (NN + 2 * VV + (size_t)2 * panel_width(c->Np) * c->Np)) * sizeof(cd);
+        const int P = slots_for(c, kfs, threads, nj, per_job);
+        if ((st = ensure_ws(c, per_job * P))) return st;
+        a.ws = (cd *)c->ws;
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        for (int base = 0; base < nj; base += P) {
+            a.job_base = base;
+            const int grid = 2 * std::min(P, nj - base);
+            for (int ph = 0; ph < 2; ++ph) {
+                a.phase = ph;
+                if (shp == SHAPE_640) hipLaunchKernelGGL((k_neq_split<640, 32, true>), dim3(grid), dim3(threads), 0, c->stream, a);
+                else if (shp == SHAPE_1024_TWO) hipLaunchKernelGGL((k_neq_split<1024, 32, true>), dim3(grid), dim3(threads), 0, c->stream, a);
+                else hipLaunchKernelGGL((k_neq_split<1024, 16, false>), dim3(grid), dim3(threads), 0, c->stream, a);
+                HIPCHK(hipGetLastError());
+            }
+        }
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        std::vector<int> h_st(2 * (size_t)nj);
+        HIPCHK(hipMemcpyAsync(density_matrix, d_out, nout * sizeof(cd), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(h_st.data(), d_st, 2 * (size_t)nj * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->last_kernel_s = ms * 1e-3;
+        for (int j = 0; j < nj; ++j)
+            if (h_st[2 * j] || h_st[2 * j + 1]) return h_st[2 * j] ? h_st[2 * j] : h_st[2 * j + 1];
+        return 0;
+    }
+    const size_t per_slot = ((size_t)(c->Nz + 1) * c->Np * c->Np + (size_t)2 * nLmax * c->Np +
+                             (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
+    const void *kfn = shp == SHAPE_640 ? (const void *)k_neq<640, 32, true>
+                    : shp == SHAPE_1024_TWO ? (const void *)k_neq<1024, 32, true>
+                                            : (const void *)k_neq<1024, 16, false>;
+    const int slots = slots_for(c, kfn, threads, nj, per_slot);
+    if ((st = ensure_ws(c, per_slot * slots))) return st;
+    a.ws = (cd *)c->ws;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     if (shp == SHAPE_640) hipLaunchKernelGGL((k_neq<640, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
     else if (shp == SHAPE_1024_TWO) hipLaunchKernelGGL((k_neq<1024, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);

@@ -11,11 +11,8 @@ import time
 from . import negf, workloads
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser(description="64-node contour density at N = 317, N_z = 1000")
-    ap.add_argument("--nodes", type=int, default=64)
-    ap.add_argument("--repeats", type=int, default=1)
-    a = ap.parse_args(argv)
+def measure(nodes=64, repeats=1, log=print):
+    """Device time of the density call (best of `repeats`), as a dict."""
     wl = workloads.disorder_c1()
     NGX, NGY = wl.ngx()
     vrec = negf.local_potential_real_to_planewave(wl.V_real_eV / negf.HARTREE, NGX, NGY)
@@ -24,16 +21,27 @@ def main(argv=None):
     # Fermi level at 15 meV (m* scaled, as the workload's energies), 4 K
     at = negf.Atomic(V_L=at0.V_L, V_R=at0.V_R, ETA=at0.ETA, LX=at0.LX, LY=at0.LY, LZ=at0.LZ, ENCUT=at0.ENCUT,
                      MU=wl.mstar * 0.015 / negf.HARTREE, TEMPERATURE=4.0, GAP=6 * negf.K_B * 4.0)
-    z, w = negf.density_contour(at, a.nodes, 0, float(wl.V_real_eV.min() / negf.HARTREE), "gauss-legendre")
+    z, w = negf.density_contour(at, nodes, 0, float(wl.V_real_eV.min() / negf.HARTREE), "gauss-legendre")
     times = []
-    for r in range(a.repeats):
+    for r in range(repeats):
         t0 = time.time()
         dev.density_matrix(z, w)
         times.append((dev.last_kernel_seconds, time.time() - t0))
-        print(f"# repeat {r}: device {times[-1][0]:.3f} s", flush=True)
+        log(f"# repeat {r}: device {times[-1][0]:.3f} s")
     best = min(t[0] for t in times)
-    print(json.dumps(dict(N=dev.N, N_z=dev.N_z, nodes=len(z), device_s=best, nodes_per_s=len(z) / best,
-                          status=dev.last_status)), flush=True)
+    res = dict(N=dev.N, N_z=dev.N_z, nodes=len(z), device_s=best, nodes_per_s=len(z) / best,
+               status=dev.last_status)
+    dev.close()
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="64-node contour density at N = 317, N_z = 1000")
+    ap.add_argument("--nodes", type=int, default=64)
+    ap.add_argument("--repeats", type=int, default=1)
+    a = ap.parse_args(argv)
+    res = measure(a.nodes, a.repeats, log=lambda m: print(m, flush=True))
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

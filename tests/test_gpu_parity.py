@@ -306,6 +306,35 @@ def test_neq_density_small(ng):
         assert rel_err(got, np.moveaxis(ref, 0, 2)) < RTOL
 
 
+@pytest.mark.parametrize("N_z", [2, 3, 7])
+def test_neq_two_sided_and_one_sided(ng, N_z, monkeypatch):
+    """Bias window with two workgroups per job (forward generators and the left parts'
+    first columns on one side, backward generators on the other, outwards from the cut
+    at N_z / 2) and with NEGF_DIAG_ONE_SIDED=1 (k_neq), both against the oracle's
+    dense-inversion restatement, relative 1e-10, down to N_z = 2."""
+    s, _ = small_system(N_z=N_z, seed=41, eta=1e-4, VL=0.01, VR=0.0)
+    a = s.atomic
+    a.MU, a.TEMPERATURE = 0.5, 300.0
+    a.GAP = 6 * O.K_B * a.TEMPERATURE
+    ha = ng.Atomic(MU=a.MU, TEMPERATURE=a.TEMPERATURE, GAP=a.GAP)
+    E, w = ng.bias_window(ha, 0.02, 5, "gauss-legendre")
+    dev = device_for(ng, s)
+    two = dev.neq_density_matrix(E, w)
+    assert dev.last_status == 0
+    monkeypatch.setenv("NEGF_DIAG_ONE_SIDED", "1")
+    dev1 = device_for(ng, s)
+    monkeypatch.delenv("NEGF_DIAG_ONE_SIDED")
+    one = dev1.neq_density_matrix(E, w)
+    assert dev1.last_status == 0
+    ref = np.zeros((s.N_z, s.N, s.N), dtype=np.complex128)
+    for e, q in zip(E, w):
+        ref += q * O.neq_spectral_blocks(s, float(e), 0, "L")
+    assert np.abs(ref).max() > 0
+    assert rel_err(two, np.moveaxis(ref, 0, 2)) < RTOL
+    assert rel_err(one, np.moveaxis(ref, 0, 2)) < RTOL
+    assert rel_err(two, one) < RTOL
+
+
 def test_pw_to_real(ng):
     L = 6.0
     nx, ny = O.planewave_basis(4.5 * (2 * np.pi / L) ** 2 / 2, L, L)
