@@ -235,6 +235,8 @@ def test_density_c1_full_size(ng):
     assert dev.last_status == 0
     ref = O.diagonal_blocks(oracle_system(wl, np.asarray(vrec), NGX, NGY), z, 0)
     assert rel_err(got, np.moveaxis(ref, 0, 2)) < RTOL
+    # a second call gives the same bits (two workgroups per job, no cross-run race)
+    assert np.array_equal(dev.density_matrix(np.array([z]), np.array([1.0 + 0j])), got)
 
 
 # ------------------------------------------------------------------ a12, a13, a14
@@ -333,6 +335,32 @@ def test_neq_two_sided_and_one_sided(ng, N_z, monkeypatch):
     assert rel_err(two, np.moveaxis(ref, 0, 2)) < RTOL
     assert rel_err(one, np.moveaxis(ref, 0, 2)) < RTOL
     assert rel_err(two, one) < RTOL
+
+
+def test_results_bit_reproducible(ng):
+    """Every path returns bit-identical results when called twice on the same inputs:
+    transmission (per-job results summed on the host), and the two-workgroup D mode
+    (density, LDOS) and bias window with one job per call.  Within one job every output
+    element is written by one workgroup, so any difference would point at a race.  (With
+    several jobs the FP64 atomic accumulation into shared output blocks is order-dependent
+    in the last bits, which is expected.)"""
+    s, _ = small_system(N_z=9, seed=51, eta=1e-4, VL=0.01, VR=0.0)
+    a = s.atomic
+    a.MU, a.TEMPERATURE = 0.5, 300.0
+    a.GAP = 6 * O.K_B * a.TEMPERATURE
+    dev = device_for(ng, s)
+    E = np.linspace(0.05, 1.2, 9)
+    zs = np.array([0.3 + 0.05j, 0.9 + 0.002j])
+    ha = ng.Atomic(MU=a.MU, TEMPERATURE=a.TEMPERATURE, GAP=a.GAP)
+    Eb, wb = ng.bias_window(ha, 0.02, 5, "gauss-legendre")
+    mid = len(Eb) // 2
+    runs = []
+    for _ in range(2):
+        runs.append((dev.transmission(E), dev.density_matrix(zs[1:], np.ones(1, dtype=complex)),
+                     dev.ldos_matrix(E[4:5]), dev.neq_density_matrix(Eb[mid:mid + 1], wb[mid:mid + 1])))
+    assert np.abs(runs[0][3]).max() > 0
+    for x, y in zip(*runs):
+        assert np.array_equal(x, y)
 
 
 def test_pw_to_real(ng):
