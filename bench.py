@@ -196,7 +196,7 @@ def main():
         line["cpu_baseline"]["max_abs_dT_vs_gpu"] = float(np.max(np.abs(np.array(taus) - gpu_tau)))
         line["other_configs"] = {"qpc_c0": qpc_c0_line(np, negf, workloads, workers)}
     if dev64 is not None:
-        # the same device's equilibrium density: 64 Gauss-Legendre arc nodes, one call
+        # the same device's equilibrium density: 64 Simpson arc nodes, one call
         from negftrans_amd import density_c1
         dev64.close()
         d = density_c1.measure(64, 1, log=lambda m: print(m, file=sys.stderr, flush=True))

@@ -1,7 +1,8 @@
 """Equilibrium density of the metric configuration (N = 317, N_z = 1000, disorder_c1)
-with the 64-node Gauss-Legendre arc contour, as one negf_density_batch launch on one
-GPU.  Prints one JSON line (device time from HIP events, nodes/s, status).
-`python -m negftrans_amd.density_c1 [--nodes 64]`."""
+with 64 contour nodes on the arc (the reference's Simpson rule, N_circle = 64, by
+default; Gauss-Legendre as the extension), as one negf_density_batch call on one GPU.
+Prints one JSON line (device time from HIP events, nodes/s, status).
+`python -m negftrans_amd.density_c1 [--nodes 64] [--quadrature simpson|gauss-legendre]`."""
 from __future__ import annotations
 
 import argparse
@@ -11,7 +12,7 @@ import time
 from . import negf, workloads
 
 
-def measure(nodes=64, repeats=1, log=print):
+def measure(nodes=64, repeats=1, log=print, quadrature="simpson"):
     """Device time of the density call (best of `repeats`), as a dict."""
     wl = workloads.disorder_c1()
     NGX, NGY = wl.ngx()
@@ -21,7 +22,7 @@ def measure(nodes=64, repeats=1, log=print):
     # Fermi level at 15 meV (m* scaled, as the workload's energies), 4 K
     at = negf.Atomic(V_L=at0.V_L, V_R=at0.V_R, ETA=at0.ETA, LX=at0.LX, LY=at0.LY, LZ=at0.LZ, ENCUT=at0.ENCUT,
                      MU=wl.mstar * 0.015 / negf.HARTREE, TEMPERATURE=4.0, GAP=6 * negf.K_B * 4.0)
-    z, w = negf.density_contour(at, nodes, 0, float(wl.V_real_eV.min() / negf.HARTREE), "gauss-legendre")
+    z, w = negf.density_contour(at, nodes, 0, float(wl.V_real_eV.min() / negf.HARTREE), quadrature)
     times = []
     for r in range(repeats):
         t0 = time.time()
@@ -29,8 +30,8 @@ def measure(nodes=64, repeats=1, log=print):
         times.append((dev.last_kernel_seconds, time.time() - t0))
         log(f"# repeat {r}: device {times[-1][0]:.3f} s")
     best = min(t[0] for t in times)
-    res = dict(N=dev.N, N_z=dev.N_z, nodes=len(z), device_s=best, nodes_per_s=len(z) / best,
-               status=dev.last_status)
+    res = dict(N=dev.N, N_z=dev.N_z, nodes=len(z), quadrature=quadrature, device_s=best,
+               nodes_per_s=len(z) / best, status=dev.last_status)
     dev.close()
     return res
 
@@ -39,8 +40,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description="64-node contour density at N = 317, N_z = 1000")
     ap.add_argument("--nodes", type=int, default=64)
     ap.add_argument("--repeats", type=int, default=1)
+    ap.add_argument("--quadrature", choices=("simpson", "gauss-legendre"), default="simpson")
     a = ap.parse_args(argv)
-    res = measure(a.nodes, a.repeats, log=lambda m: print(m, flush=True))
+    res = measure(a.nodes, a.repeats, log=lambda m: print(m, flush=True), quadrature=a.quadrature)
     print(json.dumps(res), flush=True)
 
 
