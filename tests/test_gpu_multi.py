@@ -157,7 +157,7 @@ def test_multi_large_block_valu_path(ng):
     np.testing.assert_allclose(single, O.transmission(big, E[0]), rtol=RTOL)
 
 
-def test_multi_survey_sizes_vs_single(ng, variant):
+def test_multi_uneven_block_sizes_vs_single(ng, variant):
     """The uneven-batch block sizes N in {61, 113, 197, 317, 509} with N_z from 40 to
     300, 12 energies each, autotuned: every device matches its single-device run to
     relative 1e-10 (bit for bit where the matrix-core variant was chosen)."""
