@@ -36,13 +36,37 @@ def measure(nodes=64, repeats=1, log=print, quadrature="simpson"):
     return res
 
 
+def measure_bias(nodes=32, vds_meV=4.0, log=print):
+    """Device time of the bias-window blocks (negf_neq_density_batch) of the same device:
+    `nodes` Gauss-Legendre nodes over [mu_R - GAP, MU + GAP] at VDS = vds_meV (m* scaled as
+    the workload's energies), MU = 15 meV, 4 K."""
+    wl = workloads.disorder_c1()
+    NGX, NGY = wl.ngx()
+    vrec = negf.local_potential_real_to_planewave(wl.V_real_eV / negf.HARTREE, NGX, NGY)
+    dev = negf.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY)
+    ha = negf.Atomic(MU=wl.mstar * 0.015 / negf.HARTREE, TEMPERATURE=4.0, GAP=6 * negf.K_B * 4.0)
+    E, w = negf.bias_window(ha, wl.mstar * vds_meV * 1e-3 / negf.HARTREE, nodes, "gauss-legendre")
+    t0 = time.time()
+    dev.neq_density_matrix(E, w)
+    res = dict(N=dev.N, N_z=dev.N_z, bias_nodes=len(E), device_s=dev.last_kernel_seconds,
+               wall_s=time.time() - t0, nodes_per_s=len(E) / dev.last_kernel_seconds, status=dev.last_status)
+    log(f"# bias window: {len(E)} nodes, device {res['device_s']:.3f} s")
+    dev.close()
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description="64-node contour density at N = 317, N_z = 1000")
     ap.add_argument("--nodes", type=int, default=64)
     ap.add_argument("--repeats", type=int, default=1)
     ap.add_argument("--quadrature", choices=("simpson", "gauss-legendre"), default="simpson")
+    ap.add_argument("--bias-nodes", type=int, default=0,
+                    help="instead: time the bias-window blocks with this many nodes (VDS = 4 meV)")
     a = ap.parse_args(argv)
-    res = measure(a.nodes, a.repeats, log=lambda m: print(m, flush=True), quadrature=a.quadrature)
+    if a.bias_nodes:
+        res = measure_bias(a.bias_nodes, log=lambda m: print(m, flush=True))
+    else:
+        res = measure(a.nodes, a.repeats, log=lambda m: print(m, flush=True), quadrature=a.quadrature)
     print(json.dumps(res), flush=True)
 
 
