@@ -54,7 +54,6 @@ static const double PI = 3.141592653589793;
 // g_m (the open rows of the running product, n_L x N) instead of the full N x N product:
 // R <- R g_m costs n_L N^2 per slice and is exact (no change of the arithmetic that
 // reaches the trace), while the inversion keeps its full N^3 cost.
-#define NEGF_RA 8   // rows of R accumulated per pass of the row product
 
 struct TransArgs {
     SysView sv;           // the device of every job, unless svs is set
@@ -77,48 +76,43 @@ struct TransArgs {
     int *status;          // per job: first zero pivot (1-based slice) or 0
 };
 
-struct RowProdShared {
-    cd Rs[NEGF_RA * 64];
-};
-
-// Rn(a, c) = sum_k R(a, k) S(k, sig(c)), a < nL, c < Np (R, Rn: ld nLmax).
-__device__ __forceinline__ void row_product(cd *__restrict__ Rn, const cd *__restrict__ R, int nL,
-                                            int ldR, const cd *__restrict__ S, const int *sig,
-                                            int Np, RowProdShared &rs) {
-    // column c of the result belongs to a lane pair (two threads per column when the
-    // workgroup has 2 Np threads, else one); lane h of the pair sums the k = h mod 2 terms
-    const bool pair = (int)blockDim.x >= 2 * Np;
-    const int c = pair ? (threadIdx.x >> 1) : threadIdx.x, h = pair ? (threadIdx.x & 1) : 0;
-    const int step = pair ? 2 : 1;
-    const cd *scol = S + (size_t)(c < Np ? sig[c] : 0) * Np;
-    for (int a0 = 0; a0 < nL; a0 += NEGF_RA) {
-        cd acc[NEGF_RA];
+// Rn(a, c) = sum_k R(a, k) S(k, sig(c)), a < nL, c < Np (R, Rn: ld ldR) on the FP64
+// matrix cores (v_mfma_f64_16x16x4f64, four real products per complex one).  Every wave
+// takes 16-column tiles of Rn, 16 rows of R at a time: A(i, k) = R(a0 + i, k0 + k) in
+// lane i + 16 k (zero for a0 + i >= nL), B(k, j) = S(k0 + k, sig(c0 + j)) in lane
+// j + 16 k, so one load instruction reads four consecutive rows of 16 columns and the
+// next k-step reuses the same cache lines.  D(i, j) comes back in lane j + 16 (i % 4),
+// register i / 4.  Np % 16 == 0.
+__device__ __forceinline__ void row_product_mfma(cd *__restrict__ Rn, const cd *__restrict__ R, int nL, int ldR,
+                                                 const cd *__restrict__ S, const int *sig, int Np) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int a0 = 0; a0 < nL; a0 += 16) {
+        const bool arow = a0 + li < nL;
+        const cd *ra = R + (a0 + li);
+        for (int c0 = 16 * w; c0 < Np; c0 += 16 * nw) {
+            const cd *sc = S + (size_t)sig[c0 + li] * Np + lk;
+            negf_d4 re0 = {0.0, 0.0, 0.0, 0.0}, re1 = re0, im0 = re0, im1 = re0;
+            for (int k0 = 0; k0 < Np; k0 += 16) {
+                cd av[4], bv[4];
 #pragma unroll
-        for (int a = 0; a < NEGF_RA; ++a) acc[a] = cmk(0.0, 0.0);
-        for (int k0 = 0; k0 < Np; k0 += 64) {
-            for (int e = threadIdx.x; e < NEGF_RA * 64; e += blockDim.x) {
-                const int a = e % NEGF_RA, k = k0 + e / NEGF_RA;
-                rs.Rs[e] = (a0 + a < nL && k < Np) ? R[(a0 + a) + (size_t)k * ldR] : cmk(0.0, 0.0);
-            }
-            __syncthreads();
-            if (c < Np) {
-                const int kend = min(64, Np - k0);
-                for (int kk = h; kk < kend; kk += step) {
-                    const cd sv = scol[k0 + kk];
+                for (int u = 0; u < 4; ++u) {
+                    bv[u] = sc[k0 + 4 * u];
+                    av[u] = arow ? ra[(size_t)(k0 + 4 * u + lk) * ldR] : cmk(0.0, 0.0);
+                }
 #pragma unroll
-                    for (int a = 0; a < NEGF_RA; ++a) acc[a] = cfma(rs.Rs[a + kk * NEGF_RA], sv, acc[a]);
+                for (int u = 0; u < 4; ++u) {
+                    re0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].x, re0, 0, 0, 0);
+                    re1 = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[u].y, bv[u].y, re1, 0, 0, 0);
+                    im0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].y, im0, 0, 0, 0);
+                    im1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].y, bv[u].x, im1, 0, 0, 0);
                 }
             }
-            __syncthreads();
-        }
-        if (pair) {
 #pragma unroll
-            for (int a = 0; a < NEGF_RA; ++a) acc[a] = cadd(acc[a], lane_swap(acc[a]));
-        }
-        if (c < Np && h == 0) {
-#pragma unroll
-            for (int a = 0; a < NEGF_RA; ++a)
-                if (a0 + a < nL) Rn[(a0 + a) + (size_t)c * ldR] = acc[a];
+            for (int r = 0; r < 4; ++r) {
+                const int a = a0 + 4 * r + lk;
+                if (a < nL) Rn[a + (size_t)(c0 + li) * ldR] = cmk(re0[r] + re1[r], im0[r] + im1[r]);
+            }
         }
     }
     __syncthreads();
@@ -142,7 +136,6 @@ __device__ __forceinline__ int gje_any(cd *Z, int Np, SH &s, cd *scratch) {
 template <int MAXT, int PB, bool TWO, bool MULTI = false, int MAXNP = NEGF_MAXNP4, int TCH = NEGF_TCH>
 __global__ __launch_bounds__(MAXT) void k_transmission(TransArgs a) {
     __shared__ GjeSh<PB, TWO, MAXNP, TCH> s;
-    __shared__ RowProdShared rs;
     const int Np = a.sv.Np, ldR = a.nLmax;
     const size_t NN = (size_t)Np * Np;
     cd *base = a.ws + (size_t)blockIdx.x * (2 * NN + 2 * (size_t)ldR * Np + 2 * (size_t)PB * Np);
@@ -183,7 +176,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission(TransArgs a) {
                 }
                 __syncthreads();
             } else if (m >= 2) {
-                row_product(Rn, R, nL, ldR, M, s.sigma, Np, rs);    // R <- R g_m
+                row_product_mfma(Rn, R, nL, ldR, M, s.sigma, Np);   // R <- R g_m
                 cd *t = R;
                 R = Rn;
                 Rn = t;
