@@ -28,7 +28,7 @@ N_PW, N_Z = 317, 1000
 # HBM bytes per energy point of the dominant kernel: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE over one 256-energy launch, scaled by a 1 GiB copy's counts (FETCH x2.0,
 # WRITE x1.0); the command lines and raw counts are in profiles/.
-TRAFFIC_BYTES_PER_POINT = {"fp64": 4.283e10, "mixed": 3.028e10}
+TRAFFIC_BYTES_PER_POINT = {"fp64": 4.298e10, "mixed": 3.028e10}
 
 
 def flops_per_point(N, Nz):
