@@ -700,6 +700,50 @@ __global__ __launch_bounds__(MAXT) void k_diagonal_split(DiagArgs a) {
     if (threadIdx.x == 0 && zp && *status == 0) *status = c;
 }
 
+// Vo(:, q) = add(:, q) - M(:, sig) Vi(:, q) for the nq columns q (add == nullptr: -M Vi);
+// V blocks are Np x nq with column stride Np.  FP64 matrix cores: every wave takes 16-row
+// tiles of Vo and 16 columns at a time; A(i, k) = M(r0 + i, sig(k0 + k)) in lane i + 16 k
+// (16 consecutive rows of one column per load), B(k, j) = Vi(k0 + k, q0 + j) in lane
+// j + 16 k (zero for q0 + j >= nq).  D(i, j) comes back in lane j + 16 (i % 4), register
+// i / 4.  M is read once per product, not once per column.  Np % 16 == 0.
+__device__ __forceinline__ void colprod_mfma(cd *__restrict__ Vo, const cd *__restrict__ M, const int *sig,
+                                             const cd *__restrict__ Vi, const cd *add, int nq, int Np) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int q0 = 0; q0 < nq; q0 += 16) {
+        const bool bcol = q0 + li < nq;
+        const cd *vb = Vi + (size_t)(q0 + li) * Np + lk;
+        for (int r0 = 16 * w; r0 < Np; r0 += 16 * nw) {
+            const cd *ma = M + (r0 + li);
+            negf_d4 re0 = {0.0, 0.0, 0.0, 0.0}, re1 = re0, im0 = re0, im1 = re0;
+            for (int k0 = 0; k0 < Np; k0 += 16) {
+                cd av[4], bv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int kk = k0 + 4 * u + lk;
+                    av[u] = ma[(size_t)(sig ? sig[kk] : kk) * Np];
+                    bv[u] = bcol ? vb[k0 + 4 * u] : cmk(0.0, 0.0);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    re0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].x, re0, 0, 0, 0);
+                    re1 = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[u].y, bv[u].y, re1, 0, 0, 0);
+                    im0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].y, im0, 0, 0, 0);
+                    im1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].y, bv[u].x, im1, 0, 0, 0);
+                }
+            }
+            if (bcol) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const size_t e = (size_t)(q0 + li) * Np + r0 + 4 * r + lk;
+                    const cd p = cmk(re0[r] + re1[r], im0[r] + im1[r]);
+                    Vo[e] = add ? csub(add[e], p) : cmk(-p.x, -p.y);
+                }
+            }
+        }
+    }
+}
+
 // Bias-window spectral blocks for the non-equilibrium density (extension, no reference
 // counterpart; the reference rejects VDS != 0, src/main.f90:129-132).  Per job the
 // backward generators are stored as in D mode, then the first block column on the open
@@ -778,14 +822,7 @@ __global__ __launch_bounds__(MAXT) void k_neq(NeqArgs a) {
         __syncthreads();
         accumulate(0, V0);
         for (int k = 2; k <= Nz; ++k) {
-            const cd *gk = G + (size_t)(k - 1) * NN;
-            for (int e = threadIdx.x; e < nL * Np; e += blockDim.x) {
-                const int q = e / Np, r = e % Np;
-                const cd *vq = V0 + (size_t)q * Np;
-                cd acc = cmk(0.0, 0.0);
-                for (int c = 0; c < Np; ++c) acc = cfma(gk[r + (size_t)c * Np], vq[c], acc);
-                V1[e] = cmk(-acc.x, -acc.y);
-            }
+            colprod_mfma(V1, G + (size_t)(k - 1) * NN, nullptr, V0, nullptr, nL, Np);
             __syncthreads();
             accumulate(k - 1, V1);
             __syncthreads();
@@ -837,13 +874,7 @@ __global__ __launch_bounds__(MAXT) void k_neq_split(NeqArgs a) {
     jv.sigR = a.sigR + (size_t)job * Np;
     // Vo = add - M(:, sig) Vi on the nL open columns (add == nullptr: -M(:, sig) Vi)
     auto matvec = [&](cd *Vo, const cd *M, const int *sig, const cd *Vi, const cd *add) {
-        for (int e = threadIdx.x; e < nL * Np; e += blockDim.x) {
-            const int q = e / Np, r = e % Np;
-            const cd *vq = Vi + (size_t)q * Np;
-            cd acc = cmk(0.0, 0.0);
-            for (int c = 0; c < Np; ++c) acc = cfma(M[r + (size_t)(sig ? sig[c] : c) * Np], vq[c], acc);
-            Vo[e] = add ? csub(add[e], acc) : cmk(-acc.x, -acc.y);
-        }
+        colprod_mfma(Vo, M, sig, Vi, add, nL, Np);
     };
     if (a.phase == 0) {
         int bad = 0;
