@@ -123,9 +123,11 @@ __device__ __forceinline__ void row_product_mfma(cd *__restrict__ Rn, const cd *
 template <int PB, bool TWO, int MAXNP = NEGF_MAXNP4, int TCH = NEGF_TCH>
 using GjeSh = typename std::conditional<TWO, Gje4Shared<PB, double, MAXNP, TCH>, GjeShared<PB>>::type;
 
+// (TWO with 16-column panels: the four-tile form for 512 < Np <= 1024, gje_invert4n)
 template <int PB, bool TWO, class SH>
 __device__ __forceinline__ int gje_any(cd *Z, int Np, SH &s, cd *scratch) {
-    if constexpr (TWO) return gje_invert4<PB, double, SH>(Z, Np, s, scratch);
+    if constexpr (TWO && PB == 16) return gje_invert4n<16, 4, double, SH>(Z, Np, s, scratch);
+    else if constexpr (TWO) return gje_invert4<PB, double, SH>(Z, Np, s, scratch);
     else return gje_invert2<PB>(Z, Np, s, scratch);
 }
 
@@ -1473,14 +1475,14 @@ int slots_for(negf_ctx *c, const void *kernel, int threads, int njobs, size_t pe
 
 // FP64 transmission kernel for padded size Np: the matrix-core Gauss-Jordan with its
 // shared state sized for Np <= 64 (5 workgroups per CU), Np <= 128 (3 per CU) or up to
-// 512 (1 per CU), and the one-thread-per-row form above 512.
+// 512 (1 per CU), and above 512 its four-tile, 16-column form (ceil(Np / 64) waves).
 template <bool MULTI>
 static const void *trans_kernel(int Np) {
     if (Np <= 64) return (const void *)k_transmission<128, 32, true, MULTI, 64, 32>;
     if (Np <= 128) return (const void *)k_transmission<256, 32, true, MULTI, 128, 64>;
     if (Np <= 320) return (const void *)k_transmission<640, 32, true, MULTI>;
     if (Np <= 512) return (const void *)k_transmission<1024, 32, true, MULTI>;
-    return (const void *)k_transmission<1024, 16, false, MULTI>;
+    return (const void *)k_transmission<1024, 16, true, MULTI, NEGF_MAXNP>;
 }
 template <bool MULTI>
 static void launch_trans(int Np, int slots, int threads, hipStream_t st, const TransArgs &a) {
@@ -1488,7 +1490,7 @@ static void launch_trans(int Np, int slots, int threads, hipStream_t st, const T
     else if (Np <= 128) hipLaunchKernelGGL((k_transmission<256, 32, true, MULTI, 128, 64>), dim3(slots), dim3(threads), 0, st, a);
     else if (Np <= 320) hipLaunchKernelGGL((k_transmission<640, 32, true, MULTI>), dim3(slots), dim3(threads), 0, st, a);
     else if (Np <= 512) hipLaunchKernelGGL((k_transmission<1024, 32, true, MULTI>), dim3(slots), dim3(threads), 0, st, a);
-    else hipLaunchKernelGGL((k_transmission<1024, 16, false, MULTI>), dim3(slots), dim3(threads), 0, st, a);
+    else hipLaunchKernelGGL((k_transmission<1024, 16, true, MULTI, NEGF_MAXNP>), dim3(slots), dim3(threads), 0, st, a);
 }
 
 static void launch_inverse(int Np, int count, hipStream_t st, int N, const cd *A, const cd *Sub, cd *work,
@@ -2493,8 +2495,8 @@ int negf_vhat_from_real(int N_x, int N_y, int N_z, const double *V_real, int NGX
 namespace {
 
 enum Variant { VAR_MFMA = 0, VAR_VALU = 1 };
-bool variant_ok(int v, int Np) { return v == VAR_VALU ? Np <= NEGF_MAXNP : Np <= 512; }
-int variant_threads(int v, int Np) { return v == VAR_VALU ? std::min(1024, round_up(Np, 64)) : round_up(2 * Np, 64); }
+bool variant_ok(int v, int Np) { (void)v; return Np <= NEGF_MAXNP; }
+int variant_threads(int v, int Np) { return v == VAR_VALU ? std::min(1024, round_up(Np, 64)) : threads_for_shape(Np); }
 int variant_pb(int v) { return v == VAR_VALU ? 16 : 32; }
 const void *variant_kernel(int v, int Np) {
     if (v == VAR_VALU) return (const void *)k_transmission<1024, 16, false, true>;

@@ -970,3 +970,266 @@ __device__ __forceinline__ int gje_invert4(typename GjePrec<R>::C *__restrict__ 
     __syncthreads();
     return s.flag;
 }
+
+// ====================================================================================
+// The same matrix-core Gauss-Jordan for 512 < Np <= 1024: every wave owns NT = 4 row
+// tiles (64 rows; ceil(Np / 64) <= 16 waves, so at most 1024 threads) and the panel is
+// PB = 16 columns wide, which keeps the panel registers at NT * PB / 4 = 16 complex values
+// per lane, as the two-tile, 32-column form has.  Lane l = j + 16 k holds
+// P[t][q] = Z[R0 + 16 t + j][k0 + 4 q + k]; tiles at or past Np are skipped.  Pivot rule,
+// interchanges, column map and the trailing update's MFMA layout are those of
+// gje_invert4; the update runs one row tile at a time.
+// ====================================================================================
+template <int PB, int NT, int KJ, class R, class SH>
+__device__ __forceinline__ void gje4n_step(typename GjePrec<R>::C (&P)[NT][PB / 4], const int g, const int k0,
+                                           const int Np, const int w, const int j, const int k, SH &s,
+                                           const int R0, const int ntv) {
+    typedef typename GjePrec<R>::C C;
+    constexpr int KK = PB / 4;
+    const int t = threadIdx.x, lane = t & 63, nw = blockDim.x >> 6;
+    const int J = 4 * g + KJ, c = k0 + J, par = J & 1;
+    const bool mine = (k == KJ);
+    R v[NT];
+#pragma unroll
+    for (int u = 0; u < NT; ++u) v[u] = R(-1);
+    if (mine) {
+#pragma unroll
+        for (int u = 0; u < NT; ++u) {
+            if (u < ntv) {
+                const int r = R0 + 16 * u + j;
+                if (r >= c) {
+                    v[u] = cabs1(P[u][0]);
+                    if (!(v[u] >= R(0))) v[u] = R(-0.5);
+                }
+                s.colJ[par][r] = P[u][0];
+            }
+        }
+    }
+    // izamax order: largest |re| + |im|, lowest row on ties (tiles in row order)
+    R vl = v[0];
+#pragma unroll
+    for (int u = 1; u < NT; ++u) vl = v[u] > vl ? v[u] : vl;
+    const R vmax = row16_max(vl, KJ);
+    int idx = 0x7fffffff;
+    if (vmax > R(-1)) {
+#pragma unroll
+        for (int u = NT - 1; u >= 0; --u) {
+            const unsigned long long h = __ballot(v[u] == vmax);
+            if (h) idx = R0 + 16 * u + ((__ffsll((long long)h) - 1) & 15);
+        }
+    }
+    if (lane == 0) {
+        s.rv[par][w] = vmax;
+        s.ri[par][w] = idx;
+    }
+    const bool cnd = idx != 0x7fffffff && j == (idx & 15);
+    const int ti = (idx - R0) >> 4;
+    const bool rc = c >= R0 && c < R0 + 16 * ntv && j == (c & 15);
+    const int tc = (c - R0) >> 4;
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+        if (cnd && ti == u) {
+#pragma unroll
+            for (int q = 0; q < KK; ++q) s.cand[par][w][4 * ((q + g) % KK) + k] = P[u][q];
+        }
+        if (rc && tc == u) {
+#pragma unroll
+            for (int q = 0; q < KK; ++q) s.rowC[par][4 * ((q + g) % KK) + k] = P[u][q];
+        }
+    }
+    __syncthreads();
+    // winner over the waves (at most 16, all in DPP row 0): first wave holding the max
+    const R wv = lane < nw ? s.rv[par][lane] : R(-2);
+    const int wi = lane < nw ? s.ri[par][lane] : 0x7fffffff;
+    const R bmax = row16_max(wv, 0);
+    const int wb = __ffsll((long long)__ballot(wv == bmax)) - 1;
+    int p = __builtin_amdgcn_readlane(wi, wb);
+    if (p < c || p >= Np) p = c;
+    const C *prow = (p == c) ? s.rowC[par] : s.cand[par][wb];
+    const C inv = crecip_fast(prow[J]);
+    if (t == 0) {
+        s.ipiv[c] = p;
+        if (p != c) {
+            const int tt = s.perm[c];
+            s.perm[c] = s.perm[p];
+            s.perm[p] = tt;
+        }
+        if (prow[J].x == R(0) && prow[J].y == R(0) && s.flag == 0) s.flag = c + 1;
+    }
+    // eliminate column J from the rows of this lane: the pivot row position takes the
+    // scaled pivot row, the row that supplied the pivot takes old row c
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+        if (u < ntv) {
+            const int r = R0 + 16 * u + j;
+            C m = s.colJ[par][r];
+            const bool piv = (r == c);
+            if (r == p && p != c) {
+                m = s.rowC[par][J];
+#pragma unroll
+                for (int q = 0; q < KK; ++q) P[u][q] = s.rowC[par][4 * ((q + g) % KK) + k];
+            }
+            if (piv) {
+#pragma unroll
+                for (int q = 0; q < KK; ++q) P[u][q] = cmk(R(0), R(0));
+            }
+            const C f = piv ? cmk(-inv.x, -inv.y) : cmul(m, inv);
+#pragma unroll
+            for (int q = 0; q < KK; ++q) {
+                const C pq = prow[4 * ((q + g) % KK) + k];
+                const C n = cfms(f, pq, P[u][q]);
+                P[u][q] = (q == 0 && mine) ? cmk(-f.x, -f.y) : n;
+            }
+        }
+    }
+}
+
+// In-place inversion of Z (Np x Np, 512 < Np <= 1024, Np % 32 == 0) by ceil(Np / 64)
+// waves; on return inverse[:, c] = Z[:, s.sigma[c]].  scratch: (2 PB) x Np elements.
+// Returns the 1-based column of the first exactly-zero pivot, or 0.
+template <int PB, int NT, class R = double, class SH = Gje4Shared<PB, R>>
+__device__ __forceinline__ int gje_invert4n(typename GjePrec<R>::C *__restrict__ Z, const int Np, SH &s,
+                                            typename GjePrec<R>::C *__restrict__ scratch) {
+    static_assert(PB == 16, "four 16-row tiles per wave hold one 16-column panel");
+    typedef GjePrec<R> G;
+    typedef typename G::C C;
+    typedef typename G::Acc Acc;
+    constexpr int KK = PB / 4;
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int j = lane & 15, k = lane >> 4;
+    const int R0 = w * 16 * NT;
+    const int ntv = max(0, min(NT, (Np - R0) >> 4));      // tiles of this wave inside Np
+    const size_t ld = (size_t)Np;
+    C *Tg = scratch;                      // old pivot rows: Tg[p + c PB]
+    C *Dg = scratch + (size_t)PB * Np;    // displaced rows: Dg[d + c PB]
+    if (t == 0) s.flag = 0;
+    for (int k0 = 0; k0 < Np; k0 += PB) {
+        C P[NT][KK];
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+#pragma unroll
+            for (int q = 0; q < KK; ++q)
+                P[u][q] = u < ntv ? Z[(R0 + 16 * u + j) + (size_t)(k0 + 4 * q + k) * ld] : cmk(R(0), R(0));
+        for (int i = t; i < Np; i += blockDim.x) s.perm[i] = i;
+        if (t == 0) s.ndisp = 0;
+        __syncthreads();
+#pragma unroll 1
+        for (int g = 0; g < KK; ++g) {
+            gje4n_step<PB, NT, 0, R>(P, g, k0, Np, w, j, k, s, R0, ntv);
+            gje4n_step<PB, NT, 1, R>(P, g, k0, Np, w, j, k, s, R0, ntv);
+            gje4n_step<PB, NT, 2, R>(P, g, k0, Np, w, j, k, s, R0, ntv);
+            gje4n_step<PB, NT, 3, R>(P, g, k0, Np, w, j, k, s, R0, ntv);
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                const C x = P[u][0];
+#pragma unroll
+                for (int q = 0; q < KK - 1; ++q) P[u][q] = P[u][q + 1];
+                P[u][KK - 1] = x;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+            if (u < ntv) {
+#pragma unroll
+                for (int q = 0; q < KK; ++q) Z[(R0 + 16 * u + j) + (size_t)(k0 + 4 * q + k) * ld] = P[u][q];
+            }
+        if (k == 0) {
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                const int r = R0 + 16 * u + j;
+                if (u < ntv && r >= k0 + PB && s.perm[r] != r) s.disp[atomicAdd(&s.ndisp, 1)] = r;
+            }
+        }
+        __syncthreads();
+        const int nd = s.ndisp;
+        const int rows = PB + nd;
+        {
+            constexpr int GB = 4;
+            const int total = rows * Np;
+            for (int e0 = t; e0 < total; e0 += GB * blockDim.x) {
+                C gv[GB];
+#pragma unroll
+                for (int gg = 0; gg < GB; ++gg) {
+                    const int e = e0 + gg * blockDim.x;
+                    const int q = e % rows, col = e / rows;
+                    const bool ok = e < total && !(col >= k0 && col < k0 + PB);
+                    const int sr = ok ? (q < PB ? s.perm[k0 + q] : s.perm[s.disp[q - PB]]) : 0;
+                    gv[gg] = ok ? Z[sr + col * ld] : cmk(R(0), R(0));
+                }
+#pragma unroll
+                for (int gg = 0; gg < GB; ++gg) {
+                    const int e = e0 + gg * blockDim.x;
+                    const int q = e % rows, col = e / rows;
+                    if (e < total && !(col >= k0 && col < k0 + PB)) {
+                        if (q < PB) Tg[q + (size_t)col * PB] = gv[gg];
+                        else Dg[(q - PB) + (size_t)col * PB] = gv[gg];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int e = t; e < nd * Np; e += blockDim.x) {
+            const int q = e % nd, col = e / nd;
+            if (col >= k0 && col < k0 + PB) continue;
+            Z[s.disp[q] + col * ld] = Dg[q + (size_t)col * PB];
+        }
+        __syncthreads();
+        // trailing update on the matrix cores, one chunk of SH::CH columns at a time and
+        // one row tile at a time; the pivot rows themselves start from zero
+        for (int c0 = 0; c0 < Np; c0 += SH::CH) {
+            const int c1 = min(c0 + SH::CH, Np);
+            if (c0 >= k0 && c1 <= k0 + PB) continue;       // chunk is all panel
+            for (int e = t; e < (c1 - c0) * PB; e += blockDim.x) {
+                const int pp = e % PB, cl = e / PB;
+                const int col = c0 + cl;
+                if (!(col >= k0 && col < k0 + PB)) s.Tl[pp * SH::LDT + cl] = Tg[e + (size_t)c0 * PB];
+            }
+            __syncthreads();
+            for (int ct = c0; ct < c1; ct += 16) {
+                if (ct >= k0 && ct < k0 + PB) continue;
+                const C *tl = s.Tl + k * SH::LDT + (ct - c0 + j);
+                C tv[KK];
+#pragma unroll
+                for (int q = 0; q < KK; ++q) tv[q] = tl[4 * q * SH::LDT];
+#pragma unroll
+                for (int u = 0; u < NT; ++u) {
+                    if (u < ntv) {
+                        const int rt = R0 + 16 * u;
+                        const bool zb = rt >= k0 && rt < k0 + PB;
+                        C *zc = Z + (rt + j) + (size_t)(ct + G::CQ * k) * ld;
+                        Acc ar, ai;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const C b = zb ? cmk(R(0), R(0)) : zc[(size_t)(G::CR * q) * ld];
+                            ar[q] = b.x;
+                            ai[q] = b.y;
+                        }
+#pragma unroll
+                        for (int q = 0; q < KK; ++q) {
+                            ar = G::mfma(tv[q].x, P[u][q].x, ar);
+                            ai = G::mfma(tv[q].y, P[u][q].x, ai);
+                            ar = G::mfma(-tv[q].y, P[u][q].y, ar);
+                            ai = G::mfma(tv[q].x, P[u][q].y, ai);
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) zc[(size_t)(G::CR * q) * ld] = cmk(ar[q], ai[q]);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (t == 0) {
+        for (int c = 0; c < Np; ++c) s.sigma[c] = c;
+        for (int c = Np - 1; c >= 0; --c) {
+            const int p = s.ipiv[c];
+            const int tt = s.sigma[c];
+            s.sigma[c] = s.sigma[p];
+            s.sigma[p] = tt;
+        }
+    }
+    __syncthreads();
+    return s.flag;
+}

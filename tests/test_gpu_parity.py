@@ -393,8 +393,23 @@ def test_nonlocal_potential(ng, Nx, Ny, Rx, Ry):
     assert rel_err(got, ref) < RTOL
 
 
+@pytest.mark.parametrize("R2", [191.5, 318.5])
+def test_transmission_wide_blocks(ng, R2):
+    """Blocks above Np = 512 run the four-tile, 16-column matrix-core Gauss-Jordan:
+    N = 593 (padded 608: the last wave owns two of its four tiles) and N = 997 (padded
+    1024: sixteen full waves), three slices, against the oracle, relative 1e-10."""
+    s, _ = small_system(N_z=3, seed=61, R2=R2, eta=1e-4)
+    assert s.N in (593, 997)
+    kin = O.kinetic(s.nx, s.ny, s.atomic.LX, s.atomic.LY, 0.0, 0.0)
+    E = np.array([0.3, 0.9]) * kin.max() * 0.05
+    ref = O.transmission(s, E)
+    got = device_for(ng, s).transmission(E)
+    assert np.abs(ref).max() > 0
+    np.testing.assert_allclose(got, ref, rtol=RTOL, atol=1e-12 * max(1.0, np.abs(ref).max()))
+
+
 def test_transmission_c3_block_size(ng):
-    """The largest configuration's block size (N = 797, padded 800: one thread per row,
+    """The largest configuration's block size (N = 797, padded 800: four-tile waves,
     16-column panels) on the disorder of that configuration: against the oracle on the
     first 6 slices (2 energies, relative 1e-10), and at N_z = 40 the mirror property --
     with V_L = V_R, reversing the potential in z leaves T unchanged (two-terminal
