@@ -118,10 +118,13 @@ __device__ __forceinline__ void row_product_mfma(cd *__restrict__ Rn, const cd *
     __syncthreads();
 }
 
-// TWO: two lanes per matrix row and the trailing update on the FP64 matrix cores
-// (gje_invert4, Np <= 512); otherwise one thread per row (gje_invert2).
+// TWO: the trailing update on the FP64 matrix cores -- gje_invert4 (two lanes per row,
+// 32-column panels, Np <= 512) or, with 16-column panels, gje_invert4n (four-tile waves,
+// Np <= NEGF_MAXNP, shared state sized for it); otherwise one thread per row on the VALU
+// (gje_invert2, kept as the uneven batch's VALU variant).
 template <int PB, bool TWO, int MAXNP = NEGF_MAXNP4, int TCH = NEGF_TCH>
-using GjeSh = typename std::conditional<TWO, Gje4Shared<PB, double, MAXNP, TCH>, GjeShared<PB>>::type;
+using GjeSh = typename std::conditional<TWO, Gje4Shared<PB, double, (PB == 16 ? NEGF_MAXNP : MAXNP), TCH>,
+                                        GjeShared<PB>>::type;
 
 // (TWO with 16-column panels: the four-tile form for 512 < Np <= 1024, gje_invert4n)
 template <int PB, bool TWO, class SH>
@@ -1504,7 +1507,7 @@ static void launch_inverse(int Np, int count, hipStream_t st, int N, const cd *A
         hipLaunchKernelGGL((k_inverse_batch<1024, 32, true>), dim3(count), dim3(th), 0, st, N, Np, A, Sub, work, inv, status);
         break;
     default:
-        hipLaunchKernelGGL((k_inverse_batch<1024, 16, false>), dim3(count), dim3(th), 0, st, N, Np, A, Sub, work, inv, status);
+        hipLaunchKernelGGL((k_inverse_batch<1024, 16, true>), dim3(count), dim3(th), 0, st, N, Np, A, Sub, work, inv, status);
     }
 }
 
@@ -1866,7 +1869,7 @@ static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &e
     // two workgroups per job
     const void *kfs = shp == SHAPE_640 ? (const void *)k_diagonal_split<640, 32, true>
                     : shp == SHAPE_1024_TWO ? (const void *)k_diagonal_split<1024, 32, true>
-                                            : (const void *)k_diagonal_split<1024, 16, false>;
+                                            : (const void *)k_diagonal_split<1024, 16, true>;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfs, threads, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
@@ -1883,7 +1886,7 @@ static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &e
                 a.phase = ph;
                 if (shp == SHAPE_640) hipLaunchKernelGGL((k_diagonal_split<640, 32, true>), dim3(grid), dim3(threads), 0, c->stream, a);
                 else if (shp == SHAPE_1024_TWO) hipLaunchKernelGGL((k_diagonal_split<1024, 32, true>), dim3(grid), dim3(threads), 0, c->stream, a);
-                else hipLaunchKernelGGL((k_diagonal_split<1024, 16, false>), dim3(grid), dim3(threads), 0, c->stream, a);
+                else hipLaunchKernelGGL((k_diagonal_split<1024, 16, true>), dim3(grid), dim3(threads), 0, c->stream, a);
                 HIPCHK(hipGetLastError());
             }
         }
@@ -1903,14 +1906,14 @@ static int diagonal_jobs(negf_ctx *c, const std::vector<std::complex<double>> &e
         ((size_t)(c->Nz + 2) * c->Np * c->Np + (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
     const void *kfn = shp == SHAPE_640 ? (const void *)k_diagonal<640, 32, true>
                     : shp == SHAPE_1024_TWO ? (const void *)k_diagonal<1024, 32, true>
-                                            : (const void *)k_diagonal<1024, 16, false>;
+                                            : (const void *)k_diagonal<1024, 16, true>;
     const int slots = slots_for(c, kfn, threads, nj, per_slot);
     if ((st = ensure_ws(c, per_slot * slots))) return st;
     a.ws = (cd *)c->ws;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     if (shp == SHAPE_640) hipLaunchKernelGGL((k_diagonal<640, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
     else if (shp == SHAPE_1024_TWO) hipLaunchKernelGGL((k_diagonal<1024, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
-    else hipLaunchKernelGGL((k_diagonal<1024, 16, false>), dim3(slots), dim3(threads), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_diagonal<1024, 16, true>), dim3(slots), dim3(threads), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     std::vector<int> h_st(nj);
@@ -2167,7 +2170,7 @@ int negf_neq_density_batch(negf_ctx *c, const double *E, const double *w, int nE
     // few jobs: two workgroups per job (k_neq_split), as in D mode
     const void *kfs = shp == SHAPE_640 ? (const void *)k_neq_split<640, 32, true>
                     : shp == SHAPE_1024_TWO ? (const void *)k_neq_split<1024, 32, true>
-                                            : (const void *)k_neq_split<1024, 16, false>;
+                                            : (const void *)k_neq_split<1024, 16, true>;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfs, threads, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
@@ -2186,7 +2189,7 @@ int negf_neq_density_batch(negf_ctx *c, const double *E, const double *w, int nE
                 a.phase = ph;
                 if (shp == SHAPE_640) hipLaunchKernelGGL((k_neq_split<640, 32, true>), dim3(grid), dim3(threads), 0, c->stream, a);
                 else if (shp == SHAPE_1024_TWO) hipLaunchKernelGGL((k_neq_split<1024, 32, true>), dim3(grid), dim3(threads), 0, c->stream, a);
-                else hipLaunchKernelGGL((k_neq_split<1024, 16, false>), dim3(grid), dim3(threads), 0, c->stream, a);
+                else hipLaunchKernelGGL((k_neq_split<1024, 16, true>), dim3(grid), dim3(threads), 0, c->stream, a);
                 HIPCHK(hipGetLastError());
             }
         }
@@ -2206,14 +2209,14 @@ int negf_neq_density_batch(negf_ctx *c, const double *E, const double *w, int nE
                              (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
     const void *kfn = shp == SHAPE_640 ? (const void *)k_neq<640, 32, true>
                     : shp == SHAPE_1024_TWO ? (const void *)k_neq<1024, 32, true>
-                                            : (const void *)k_neq<1024, 16, false>;
+                                            : (const void *)k_neq<1024, 16, true>;
     const int slots = slots_for(c, kfn, threads, nj, per_slot);
     if ((st = ensure_ws(c, per_slot * slots))) return st;
     a.ws = (cd *)c->ws;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     if (shp == SHAPE_640) hipLaunchKernelGGL((k_neq<640, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
     else if (shp == SHAPE_1024_TWO) hipLaunchKernelGGL((k_neq<1024, 32, true>), dim3(slots), dim3(threads), 0, c->stream, a);
-    else hipLaunchKernelGGL((k_neq<1024, 16, false>), dim3(slots), dim3(threads), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_neq<1024, 16, true>), dim3(slots), dim3(threads), 0, c->stream, a);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     std::vector<int> h_st(nj);
@@ -3103,7 +3106,7 @@ int negf_multi_ldos(negf_multi *m, const double *const *E, const int *nE, const 
             const Shape shp = shape_for(Np);
             const void *kfn = shp == SHAPE_640 ? (const void *)k_diagonal<640, 32, true, true>
                             : shp == SHAPE_1024_TWO ? (const void *)k_diagonal<1024, 32, true, true>
-                                                    : (const void *)k_diagonal<1024, 16, false, true>;
+                                                    : (const void *)k_diagonal<1024, 16, true, true>;
             int per_cu = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, threads_for_shape(Np), 0) != hipSuccess ||
                 per_cu < 1)
@@ -3164,7 +3167,7 @@ int negf_multi_ldos(negf_multi *m, const double *const *E, const int *nE, const 
             hipLaunchKernelGGL((k_diagonal<1024, 32, true, true>), dim3(g.slots), dim3(th), 0, gsm, g.a);
             break;
         default:
-            hipLaunchKernelGGL((k_diagonal<1024, 16, false, true>), dim3(g.slots), dim3(th), 0, gsm, g.a);
+            hipLaunchKernelGGL((k_diagonal<1024, 16, true, true>), dim3(g.slots), dim3(th), 0, gsm, g.a);
         }
         ok = hipGetLastError() == hipSuccess && hipEventRecord(m->gev[gi], gsm) == hipSuccess &&
              hipStreamWaitEvent(m->s0, m->gev[gi], 0) == hipSuccess;

@@ -408,6 +408,28 @@ def test_transmission_wide_blocks(ng, R2):
     np.testing.assert_allclose(got, ref, rtol=RTOL, atol=1e-12 * max(1.0, np.abs(ref).max()))
 
 
+def test_wide_blocks_d_mode_and_bias_window(ng, monkeypatch):
+    """D mode (two-sided and one-sided) and the bias window above Np = 512 (N = 593,
+    padded 608, four-tile matrix-core inversion), three slices, against the oracle,
+    relative 1e-10."""
+    s, _ = small_system(N_z=3, seed=62, R2=191.5, eta=1e-4, VL=0.01, VR=0.0)
+    assert s.N == 593
+    kin = O.kinetic(s.nx, s.ny, s.atomic.LX, s.atomic.LY, 0.0, 0.0)
+    z = complex(0.4 * 0.05 * kin.max(), 0.01)
+    ref = np.moveaxis(O.diagonal_blocks(s, z, 0), 0, 2)
+    dev = device_for(ng, s)
+    assert rel_err(dev.density_matrix(np.array([z]), np.ones(1, dtype=complex)), ref) < RTOL
+    monkeypatch.setenv("NEGF_DIAG_ONE_SIDED", "1")
+    dev1 = device_for(ng, s)
+    monkeypatch.delenv("NEGF_DIAG_ONE_SIDED")
+    assert rel_err(dev1.density_matrix(np.array([z]), np.ones(1, dtype=complex)), ref) < RTOL
+    e = 0.4 * 0.05 * kin.max()
+    refb = np.moveaxis(O.neq_spectral_blocks(s, e, 0, "L"), 0, 2)
+    assert np.abs(refb).max() > 0
+    got = dev.neq_density_matrix(np.array([e]), np.ones(1, dtype=complex))
+    assert rel_err(got, refb) < RTOL
+
+
 def test_transmission_c3_block_size(ng):
     """The largest configuration's block size (N = 797, padded 800: four-tile waves,
     16-column panels) on the disorder of that configuration: against the oracle on the
