@@ -54,6 +54,15 @@ static const double PI = 3.141592653589793;
 // g_m (the open rows of the running product, n_L x N) instead of the full N x N product:
 // R <- R g_m costs n_L N^2 per slice and is exact (no change of the arithmetic that
 // reaches the trace), while the inversion keeps its full N^3 cost.
+#define NEGF_RA 8   // rows of R accumulated per pass of the VALU row product
+// The row product runs on the matrix cores for one device at padded sizes 256..320 (the
+// VALU form's strided loads cost more there than the 16-row MFMA tiles' padding); the
+// VALU form elsewhere and in uneven batches, where the matrix-core form slowed the
+// batch by half (measured).  A batch's T therefore equals the single-device T to the
+// last bit except at those padded sizes, where the two summation orders differ at the
+// 1e-13 level.
+#define NEGF_RP_MFMA_LO 256
+#define NEGF_RP_MFMA_HI 320
 
 struct TransArgs {
     SysView sv;           // the device of every job, unless svs is set
@@ -118,6 +127,55 @@ __device__ __forceinline__ void row_product_mfma(cd *__restrict__ Rn, const cd *
     __syncthreads();
 }
 
+struct RowProdShared {
+    cd Rs[NEGF_RA * 64];
+};
+
+// The same product on the VALU, NEGF_RA rows of R at a time staged in LDS: for small
+// blocks, where the matrix-core form's 16-row tiles and MFMA latency cost more than its
+// better loads gain (Rn, R: ld ldR).
+__device__ __forceinline__ void row_product(cd *__restrict__ Rn, const cd *__restrict__ R, int nL,
+                                            int ldR, const cd *__restrict__ S, const int *sig,
+                                            int Np, RowProdShared &rs) {
+    // column c of the result belongs to a lane pair (two threads per column when the
+    // workgroup has 2 Np threads, else one); lane h of the pair sums the k = h mod 2 terms
+    const bool pair = (int)blockDim.x >= 2 * Np;
+    const int c = pair ? (threadIdx.x >> 1) : threadIdx.x, h = pair ? (threadIdx.x & 1) : 0;
+    const int step = pair ? 2 : 1;
+    const cd *scol = S + (size_t)(c < Np ? sig[c] : 0) * Np;
+    for (int a0 = 0; a0 < nL; a0 += NEGF_RA) {
+        cd acc[NEGF_RA];
+#pragma unroll
+        for (int a = 0; a < NEGF_RA; ++a) acc[a] = cmk(0.0, 0.0);
+        for (int k0 = 0; k0 < Np; k0 += 64) {
+            for (int e = threadIdx.x; e < NEGF_RA * 64; e += blockDim.x) {
+                const int a = e % NEGF_RA, k = k0 + e / NEGF_RA;
+                rs.Rs[e] = (a0 + a < nL && k < Np) ? R[(a0 + a) + (size_t)k * ldR] : cmk(0.0, 0.0);
+            }
+            __syncthreads();
+            if (c < Np) {
+                const int kend = min(64, Np - k0);
+                for (int kk = h; kk < kend; kk += step) {
+                    const cd sv = scol[k0 + kk];
+#pragma unroll
+                    for (int a = 0; a < NEGF_RA; ++a) acc[a] = cfma(rs.Rs[a + kk * NEGF_RA], sv, acc[a]);
+                }
+            }
+            __syncthreads();
+        }
+        if (pair) {
+#pragma unroll
+            for (int a = 0; a < NEGF_RA; ++a) acc[a] = cadd(acc[a], lane_swap(acc[a]));
+        }
+        if (c < Np && h == 0) {
+#pragma unroll
+            for (int a = 0; a < NEGF_RA; ++a)
+                if (a0 + a < nL) Rn[(a0 + a) + (size_t)c * ldR] = acc[a];
+        }
+    }
+    __syncthreads();
+}
+
 // TWO: the trailing update on the FP64 matrix cores -- gje_invert4 (two lanes per row,
 // 32-column panels, Np <= 512) or, with 16-column panels, gje_invert4n (four-tile waves,
 // Np <= NEGF_MAXNP, shared state sized for it); otherwise one thread per row on the VALU
@@ -141,6 +199,7 @@ __device__ __forceinline__ int gje_any(cd *Z, int Np, SH &s, cd *scratch) {
 template <int MAXT, int PB, bool TWO, bool MULTI = false, int MAXNP = NEGF_MAXNP4, int TCH = NEGF_TCH>
 __global__ __launch_bounds__(MAXT) void k_transmission(TransArgs a) {
     __shared__ GjeSh<PB, TWO, MAXNP, TCH> s;
+    __shared__ RowProdShared rs;
     const int Np = a.sv.Np, ldR = a.nLmax;
     const size_t NN = (size_t)Np * Np;
     cd *base = a.ws + (size_t)blockIdx.x * (2 * NN + 2 * (size_t)ldR * Np + 2 * (size_t)PB * Np);
@@ -181,7 +240,9 @@ __global__ __launch_bounds__(MAXT) void k_transmission(TransArgs a) {
                 }
                 __syncthreads();
             } else if (m >= 2) {
-                row_product_mfma(Rn, R, nL, ldR, M, s.sigma, Np);   // R <- R g_m
+                if (!MULTI && Np >= NEGF_RP_MFMA_LO && Np <= NEGF_RP_MFMA_HI)
+                    row_product_mfma(Rn, R, nL, ldR, M, s.sigma, Np);   // R <- R g_m
+                else row_product(Rn, R, nL, ldR, M, s.sigma, Np, rs);
                 cd *t = R;
                 R = Rn;
                 Rn = t;
