@@ -204,6 +204,11 @@ def main():
         d["achieved_tflops"] = d["flop"] / d["device_s"] / 1e12
         d["frac_of_fp64_peak"] = d["achieved_tflops"] / PEAK_FP64_TFLOPS
         line.setdefault("other_configs", {})["density_c1"] = d
+        # the largest configuration's block size (N = 797) at reduced depth, 256 energies
+        from negftrans_amd import block_c3
+        c3 = block_c3.measure(40, (256,))[0]
+        c3["note"] = "N_z = 40 of the configuration's 2000; points_per_s_at_Nz2000 scales the measured rate"
+        line["other_configs"]["block_c3"] = c3
     print(json.dumps(line), flush=True)
 
 

@@ -13,25 +13,34 @@ import numpy as np
 from . import negf, workloads
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser(description="N = 797 transmission timing")
-    ap.add_argument("--nz", type=int, default=40)
-    ap.add_argument("--energies", type=int, nargs="+", default=[64, 256])
-    a = ap.parse_args(argv)
-    wl = workloads.disorder_c3(N_z=a.nz)
+def measure(nz=40, batches=(64, 256), log=None):
+    """One dict per batch size (see the module docstring)."""
+    wl = workloads.disorder_c3(N_z=nz)
     NGX, NGY = wl.ngx()
     vrec = negf.local_potential_real_to_planewave(wl.V_real_eV / negf.HARTREE, NGX, NGY)
     dev = negf.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY)
     Eall = wl.energies_hartree()
     dev.transmission(Eall[:8])                   # warm-up
-    for n in a.energies:
+    out = []
+    for n in batches:
         E = Eall[(np.arange(n) * 3 + 500) % len(Eall)]
         T = dev.transmission(E)
         t = dev.last_kernel_seconds
-        print(json.dumps(dict(N=dev.N, N_z=dev.N_z, energies=n, device_s=t, ms_per_slice_per_job=t / dev.N_z * 1e3,
-                              points_per_s_at_Nz2000=n / (t / dev.N_z * 2000), T_max=float(T.max()),
-                              status=dev.last_status)), flush=True)
+        out.append(dict(N=dev.N, N_z=dev.N_z, energies=n, device_s=t, ms_per_slice_per_job=t / dev.N_z * 1e3,
+                        points_per_s_at_Nz2000=n / (t / dev.N_z * 2000), T_max=float(T.max()),
+                        status=dev.last_status))
+        if log:
+            log(json.dumps(out[-1]))
     dev.close()
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="N = 797 transmission timing")
+    ap.add_argument("--nz", type=int, default=40)
+    ap.add_argument("--energies", type=int, nargs="+", default=[64, 256])
+    a = ap.parse_args(argv)
+    measure(a.nz, a.energies, log=lambda m: print(m, flush=True))
 
 
 if __name__ == "__main__":
