@@ -83,6 +83,8 @@ struct TransArgs {
     cd *ws;               // per slot 2 Np^2 + 2 nLmax Np
     double *tau;          // per job
     int *status;          // per job: first zero pivot (1-based slice) or 0
+    int phase, job_base, ldX;   // k_transmission_split only: launch phase, first job,
+    int *status2;               // chain stride, per job and side first zero pivot
 };
 
 // Rn(a, c) = sum_k R(a, k) S(k, sig(c)), a < nL, c < Np (R, Rn: ld ldR) on the FP64
@@ -810,6 +812,117 @@ __device__ __forceinline__ void colprod_mfma(cd *__restrict__ Vo, const cd *__re
     }
 }
 
+// Transmission with two workgroups per job (same blocks; for calls with fewer jobs than
+// half the workgroups the GPU holds).  With the cut at m = N_z / 2 the left workgroup
+// sweeps the forward generators f_1..f_m and carries the left parts' first columns on the
+// open columns of G_{N_z,1}, L_1 = f_1(:, open), L_k = -f_k L_{k-1}; the right one sweeps
+// g_{N_z}..g_{m+1} and carries R = E^T g_{N_z} ... g_{m+1} on the open rows, as
+// k_transmission does.  Then (second launch, one workgroup per job)
+//   G_{m,1}(:, open) = (I - f_m g_{m+1})^{-1} L_m,   G_{N_z,1}(open, open) = +-R G_{m,1},
+// and tau as in k_transmission (the sign drops out of |.|^2).  Chains and generators
+// alternate between two buffers by step parity, so phase 1 finds L_m and R from m and
+// N_z alone; each side leaves its last generator unpermuted for the product.
+template <int MAXT, int PB, bool TWO, int MAXNP = NEGF_MAXNP4, int TCH = NEGF_TCH>
+__global__ __launch_bounds__(MAXT) void k_transmission_split(TransArgs a) {
+    __shared__ GjeSh<PB, TWO, MAXNP, TCH> s;
+    __shared__ RowProdShared rs;
+    const SysView &sv = a.sv;
+    const int Np = sv.Np, Nz = sv.Nz, m = Nz / 2, ldX = a.ldX;
+    const size_t NN = (size_t)Np * Np, XX = (size_t)ldX * Np, SCR = 2 * (size_t)PB * Np;
+    const size_t side_sz = 2 * NN + 2 * XX + SCR;
+    const size_t per_job = 2 * side_sz + 3 * NN + XX + SCR;
+    const int jl = a.phase == 0 ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
+    const int side = a.phase == 0 ? (int)(blockIdx.x & 1) : 0;
+    const int job = a.job_base + jl;
+    cd *J0 = a.ws + (size_t)jl * per_job;
+    cd *F = J0 + 2 * side_sz, *Gb = F + NN, *Wm = Gb + NN, *Gm = Wm + NN, *scr2 = Gm + XX;
+    const int nL = a.nL[job], nR = a.nR[job], off = a.open_off[job];
+    if (nL == 0 || nR == 0) {                 // no open channel: tau = 0 exactly
+        if (a.phase == 1 && threadIdx.x == 0) {
+            a.tau[job] = 0.0;
+            a.status[job] = 0;
+        }
+        return;
+    }
+    JobView jv;
+    jv.z = a.z[job];
+    jv.kinh = a.kinh + (size_t)a.kidx[job] * Np;
+    jv.sigL = a.sigL + (size_t)job * Np;
+    jv.sigR = a.sigR + (size_t)job * Np;
+    if (a.phase == 0) {
+        cd *sb = J0 + (size_t)side * side_sz;
+        cd *gen[2] = {sb, sb + NN};
+        cd *buf[2] = {sb + 2 * NN, sb + 2 * NN + XX};
+        cd *scratch = sb + 2 * NN + 2 * XX;
+        int bad = 0;
+        if (side == 0) {
+            for (int k = 1; k <= m; ++k) {
+                cd *M = gen[k & 1];
+                build_block(M, sv, jv, k - 1, k == 1 ? nullptr : gen[(k - 1) & 1], s.sigma);
+                __syncthreads();
+                if (gje_any<PB, TWO>(M, Np, s, scratch) && !bad) bad = k;
+                if (k == 1) {
+                    for (int e = threadIdx.x; e < nR * Np; e += blockDim.x) {
+                        const int q = e / Np, r = e % Np;
+                        buf[0][e] = M[r + (size_t)s.sigma[a.Rcol[off + q]] * Np];
+                    }
+                } else {
+                    colprod_mfma(buf[(k - 1) & 1], M, s.sigma, buf[(k - 2) & 1], nullptr, nR, Np);
+                }
+                __syncthreads();
+            }
+            copy_cols(F, gen[m & 1], s.sigma, Np);
+        } else {
+            for (int k = Nz; k > m; --k) {
+                cd *M = gen[k & 1];
+                build_block(M, sv, jv, k - 1, k == Nz ? nullptr : gen[(k + 1) & 1], s.sigma);
+                __syncthreads();
+                if (gje_any<PB, TWO>(M, Np, s, scratch) && !bad) bad = k;
+                if (k == Nz) {
+                    for (int e = threadIdx.x; e < nL * Np; e += blockDim.x) {
+                        const int ai = e % nL, c = e / nL;
+                        buf[0][ai + (size_t)c * ldX] = M[a.Lrow[off + ai] + (size_t)s.sigma[c] * Np];
+                    }
+                    __syncthreads();
+                } else {
+                    cd *Rn = buf[(Nz - k) & 1], *Ro = buf[(Nz - k - 1) & 1];
+                    if (Np >= NEGF_RP_MFMA_LO && Np <= NEGF_RP_MFMA_HI) row_product_mfma(Rn, Ro, nL, ldX, M, s.sigma, Np);
+                    else row_product(Rn, Ro, nL, ldX, M, s.sigma, Np, rs);
+                }
+            }
+            copy_cols(Gb, gen[(m + 1) & 1], s.sigma, Np);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) a.status2[2 * (size_t)job + side] = bad;
+        return;
+    }
+    // phase 1: W = I - f_m g_{m+1} on the physical block (identity on the padding)
+    const int N = sv.N;
+    gemm_nn(F, Gb, nullptr, Np, [&](int r, int c, cd v) {
+        Wm[r + (size_t)c * Np] = (r < N && c < N) ? cmk((r == c ? 1.0 : 0.0) - v.x, -v.y) : cmk(r == c ? 1.0 : 0.0, 0.0);
+    });
+    __syncthreads();
+    const int zp = gje_any<PB, TWO>(Wm, Np, s, scr2);
+    const cd *Lm = J0 + 2 * NN + (size_t)((m - 1) & 1) * XX;                       // left side
+    const cd *Rm = J0 + side_sz + 2 * NN + (size_t)((Nz - m - 1) & 1) * XX;         // right side
+    colprod_mfma(Gm, Wm, s.sigma, Lm, nullptr, nR, Np);                             // -G_{m,1}(:, open)
+    __syncthreads();
+    double part = 0.0;
+    for (int e = threadIdx.x; e < nL * nR; e += blockDim.x) {
+        const int ai = e % nL, bi = e / nL;
+        const cd *gc = Gm + (size_t)bi * Np;
+        cd acc = cmk(0.0, 0.0);
+        for (int k = 0; k < Np; ++k) acc = cfma(Rm[ai + (size_t)k * ldX], gc[k], acc);
+        part += a.gL[off + ai] * a.gR[off + bi] * (acc.x * acc.x + acc.y * acc.y);
+    }
+    const double tot = block_sum2(part, s.red);
+    if (threadIdx.x == 0) {
+        const int b0 = a.status2[2 * (size_t)job], b1 = a.status2[2 * (size_t)job + 1];
+        a.tau[job] = tot * sv.scale * sv.scale;
+        a.status[job] = b0 ? b0 : (b1 ? b1 : (zp ? m : 0));
+    }
+}
+
 // Bias-window spectral blocks for the non-equilibrium density (extension, no reference
 // counterpart; the reference rejects VDS != 0, src/main.f90:129-132).  Per job the
 // backward generators are stored as in D mode, then the first block column on the open
@@ -1329,7 +1442,7 @@ struct negf_ctx {
     int mixed_refine = 8;   // refinement iteration limit of NEGF_PREC_MIXED_FP32
     double mixed_tol = 1e-9; // its stopping tolerance (relative correction of X_{N_z})
     int last_fallbacks = 0; // energies of the last mixed call redone in FP64
-    bool diag_one_sided = false; // D mode and bias window: never the two-sided forms (NEGF_DIAG_ONE_SIDED=1)
+    bool diag_one_sided = false; // transmission, D mode, bias window: never the two-sided forms (NEGF_DIAG_ONE_SIDED=1)
 };
 
 namespace {
@@ -1547,6 +1660,21 @@ static const void *trans_kernel(int Np) {
     if (Np <= 320) return (const void *)k_transmission<640, 32, true, MULTI>;
     if (Np <= 512) return (const void *)k_transmission<1024, 32, true, MULTI>;
     return (const void *)k_transmission<1024, 16, true, MULTI, NEGF_MAXNP>;
+}
+// The two-sided transmission kernel for padded size Np (the same shapes as trans_kernel).
+static const void *trans_split_kernel(int Np) {
+    if (Np <= 64) return (const void *)k_transmission_split<128, 32, true, 64, 32>;
+    if (Np <= 128) return (const void *)k_transmission_split<256, 32, true, 128, 64>;
+    if (Np <= 320) return (const void *)k_transmission_split<640, 32, true>;
+    if (Np <= 512) return (const void *)k_transmission_split<1024, 32, true>;
+    return (const void *)k_transmission_split<1024, 16, true, NEGF_MAXNP>;
+}
+static void launch_trans_split(int Np, int grid, int threads, hipStream_t st, const TransArgs &a) {
+    if (Np <= 64) hipLaunchKernelGGL((k_transmission_split<128, 32, true, 64, 32>), dim3(grid), dim3(threads), 0, st, a);
+    else if (Np <= 128) hipLaunchKernelGGL((k_transmission_split<256, 32, true, 128, 64>), dim3(grid), dim3(threads), 0, st, a);
+    else if (Np <= 320) hipLaunchKernelGGL((k_transmission_split<640, 32, true>), dim3(grid), dim3(threads), 0, st, a);
+    else if (Np <= 512) hipLaunchKernelGGL((k_transmission_split<1024, 32, true>), dim3(grid), dim3(threads), 0, st, a);
+    else hipLaunchKernelGGL((k_transmission_split<1024, 16, true, NEGF_MAXNP>), dim3(grid), dim3(threads), 0, st, a);
 }
 template <bool MULTI>
 static void launch_trans(int Np, int slots, int threads, hipStream_t st, const TransArgs &a) {
@@ -1849,6 +1977,55 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
             if (h_st[j] > 0 && !status) status = h_st[j];
         }
         return status;
+    }
+    a.z = d_z;
+    a.kidx = d_k;
+    a.kinh = d_kin;
+    a.sigL = d_sl;
+    a.sigR = d_sr;
+    a.open_off = d_off;
+    a.nL = d_nL;
+    a.nR = d_nR;
+    a.Lrow = d_Lr;
+    a.Rcol = d_Rc;
+    a.gL = d_gl;
+    a.gR = d_gr;
+    a.nLmax = nLmax;
+    a.tau = d_tau;
+    a.status = d_st;
+    // Few jobs (at most half the workgroups the GPU holds at once): the two-sided form,
+    // two workgroups per job in the first launch, one in the second
+    {
+        const void *kfs = trans_split_kernel(c->Np);
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfs, threads, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        if (c->Nz >= 2 && 2L * nj <= (long)c->n_cu * per_cu && !c->diag_one_sided) {
+            int ldX = 1;
+            for (int j = 0; j < nj; ++j) ldX = std::max(ldX, std::max(nL[j], nR[j]));
+            const size_t NN = (size_t)c->Np * c->Np, XX = (size_t)ldX * c->Np,
+                         SCR = (size_t)2 * panel_width(c->Np) * c->Np;
+            const size_t per_job = (2 * (2 * NN + 2 * XX + SCR) + 3 * NN + XX + SCR) * sizeof(cd);
+            int *d_st2;
+            if ((st = tmp.alloc(&d_st2, 2 * (size_t)nj))) return st;
+            const int P = slots_for(c, kfs, threads, nj, per_job);
+            if ((st = ensure_ws(c, per_job * P))) return st;
+            a.ws = (cd *)c->ws;
+            a.ldX = ldX;
+            a.status2 = d_st2;
+            HIPCHK(hipEventRecord(c->ev0, c->stream));
+            for (int base = 0; base < nj; base += P) {
+                a.job_base = base;
+                const int n = std::min(P, nj - base);
+                for (int ph = 0; ph < 2; ++ph) {
+                    a.phase = ph;
+                    launch_trans_split(c->Np, ph == 0 ? 2 * n : n, threads, c->stream, a);
+                    HIPCHK(hipGetLastError());
+                }
+            }
+            HIPCHK(hipEventRecord(c->ev1, c->stream));
+            return finish_transmission(c, nj, d_tau, d_st, je, jk, kw, tau, tau_jobs);
+        }
     }
     const size_t per_slot =
         ((size_t)2 * c->Np * c->Np + (size_t)2 * nLmax * c->Np + (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);

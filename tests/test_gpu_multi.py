@@ -57,6 +57,14 @@ def system(R2, N_z, seed, L=12.0, NK=(1, 1), eta=1e-6, amp=0.02):
     return O.System(a, nx, ny, Vrec, NGX, NGY, kx, ky, kw)
 
 
+@pytest.fixture(autouse=True)
+def one_sided_references(monkeypatch):
+    """Single-device reference runs use the one-workgroup sweep, the form the batch runs
+    (a call with few jobs otherwise takes the two-workgroup form, whose summation order
+    differs; test_gpu_parity checks that form against the oracle and this one)."""
+    monkeypatch.setenv("NEGF_DIAG_ONE_SIDED", "1")
+
+
 def device(ng, s):
     a = s.atomic
     at = ng.Atomic(V_L=a.V_L, V_R=a.V_R, ETA=a.ETA, LX=a.LX, LY=a.LY, LZ=a.LZ, ENCUT=a.ENCUT)

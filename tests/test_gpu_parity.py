@@ -278,6 +278,33 @@ def test_diagonal_two_sided_and_one_sided(ng, N_z, monkeypatch):
     assert rel_err(two, one) < RTOL
 
 
+@pytest.mark.parametrize("N_z,NK", [(2, (1, 1)), (3, (2, 1)), (8, (1, 1)), (11, (1, 2))])
+def test_transmission_two_sided_and_one_sided(ng, N_z, NK, monkeypatch):
+    """Transmission: a call with few jobs runs two workgroups per job -- forward
+    generators and the open-column chain of G_{k,1} from the left lead, backward
+    generators and the open-row chain from the right, joined at the cut N_z / 2 through
+    (I - f_m g_{m+1})^{-1}; NEGF_DIAG_ONE_SIDED=1 keeps the one-workgroup backward sweep.
+    Both against the oracle (relative 1e-10), and against each other, down to N_z = 2;
+    energies with no open channel give 0 exactly in both."""
+    s, _ = small_system(N_z=N_z, seed=41, NK=NK, eta=1e-4)
+    E = np.concatenate([[-0.2], np.linspace(0.05, 1.2, 9)])
+    kp = (s.kx, s.ky, s.kw)
+    ref = O.transmission(s, E)
+    dev = device_for(ng, s)
+    two = dev.transmission(E, kp)
+    assert dev.last_status == 0
+    monkeypatch.setenv("NEGF_DIAG_ONE_SIDED", "1")
+    dev1 = device_for(ng, s)
+    monkeypatch.delenv("NEGF_DIAG_ONE_SIDED")
+    one = dev1.transmission(E, kp)
+    assert dev1.last_status == 0
+    tol = dict(rtol=RTOL, atol=1e-12 * max(1.0, np.abs(ref).max()))
+    np.testing.assert_allclose(two, ref, **tol)
+    np.testing.assert_allclose(one, ref, **tol)
+    np.testing.assert_allclose(two, one, **tol)
+    assert two[0] == 0.0 and one[0] == 0.0 and np.abs(ref).max() > 0.5
+
+
 def test_ldos_small(ng):
     s, V = small_system(N_z=6, seed=22, NK=(1, 2), eta=1e-3)
     E = np.array([0.2, 0.7])
