@@ -64,6 +64,11 @@ static const double PI = 3.141592653589793;
 #define NEGF_RP_MFMA_LO 256
 #define NEGF_RP_MFMA_HI 320
 
+// Workgroups of at most 256 threads (padded sizes up to 128) are held to two waves per
+// SIMD, i.e. at most 256 VGPRs + AGPRs per lane, so that a CU holds four of them (left
+// to itself the compiler takes 224 VGPRs and 36-96 AGPRs there: one wave per SIMD).
+#define NEGF_SMALL_WPE(MAXT) __attribute__((amdgpu_waves_per_eu((MAXT) <= 256 ? 2 : 1)))
+
 struct TransArgs {
     SysView sv;           // the device of every job, unless svs is set
     const SysView *svs;   // uneven batches: per device, indexed by jdev[job] (same Np for all)
@@ -199,7 +204,7 @@ __device__ __forceinline__ int gje_any(cd *Z, int Np, SH &s, cd *scratch) {
 // MAXNP / TCH: size limits of the Gauss-Jordan shared state (smaller for small Np, so
 // that several workgroups fit on a CU).
 template <int MAXT, int PB, bool TWO, bool MULTI = false, int MAXNP = NEGF_MAXNP4, int TCH = NEGF_TCH>
-__global__ __launch_bounds__(MAXT) void k_transmission(TransArgs a) {
+__global__ __launch_bounds__(MAXT) NEGF_SMALL_WPE(MAXT) void k_transmission(TransArgs a) {
     __shared__ GjeSh<PB, TWO, MAXNP, TCH> s;
     __shared__ RowProdShared rs;
     const int Np = a.sv.Np, ldR = a.nLmax;
@@ -823,7 +828,7 @@ __device__ __forceinline__ void colprod_mfma(cd *__restrict__ Vo, const cd *__re
 // alternate between two buffers by step parity, so phase 1 finds L_m and R from m and
 // N_z alone; each side leaves its last generator unpermuted for the product.
 template <int MAXT, int PB, bool TWO, int MAXNP = NEGF_MAXNP4, int TCH = NEGF_TCH>
-__global__ __launch_bounds__(MAXT) void k_transmission_split(TransArgs a) {
+__global__ __launch_bounds__(MAXT) NEGF_SMALL_WPE(MAXT) void k_transmission_split(TransArgs a) {
     __shared__ GjeSh<PB, TWO, MAXNP, TCH> s;
     __shared__ RowProdShared rs;
     const SysView &sv = a.sv;
