@@ -64,8 +64,12 @@ int negf_ctx_destroy(negf_ctx *ctx);
  * Transmission_Coefficient, src/negf.f90:115-174):
  *   tau[ie] += sum_ik kw[ik] * tau(E[ie], k = (kx[ik], ky[ik]))      (ie < nE, ik < nk)
  * E in Hartree (real; +i*eta is added as at src/negf.f90:149).  All nE*nk jobs run
- * in one persistent launch.  tau_jobs (optional, may be NULL) receives the per-job
- * tau before the k weight, ordered job = ie + nE*ik (the reference's i_job - 1).
+ * in one persistent launch; when they number at most half the workgroups the GPU
+ * holds, each job runs on two workgroups (forward and backward sweeps joined at
+ * N_z/2, a second short launch), which agrees with the one-workgroup form to rounding.
+ * NEGF_DIAG_ONE_SIDED=1 in the environment at context creation keeps one workgroup per
+ * job.  tau_jobs (optional, may be NULL) receives the per-job tau before the k weight,
+ * ordered job = ie + nE*ik (the reference's i_job - 1).
  */
 int negf_transmission_batch(negf_ctx *ctx, const double *E, int nE, const double *kx,
                             const double *ky, const double *kw, int nk, double *tau,

@@ -88,8 +88,11 @@ struct TransArgs {
     cd *ws;               // per slot 2 Np^2 + 2 nLmax Np
     double *tau;          // per job
     int *status;          // per job: first zero pivot (1-based slice) or 0
-    int phase, job_base, ldX;   // k_transmission_split only: launch phase, first job,
-    int *status2;               // chain stride, per job and side first zero pivot
+    // k_transmission_split only: launch phase (0: both sweeps, 1: join), first job of
+    // the launch, column stride of the open-channel chains, and per job and side the
+    // first zero pivot of that side's sweep
+    int phase, job_base, ldX;
+    int *status2;
 };
 
 // Rn(a, c) = sum_k R(a, k) S(k, sig(c)), a < nL, c < Np (R, Rn: ld ldR) on the FP64
