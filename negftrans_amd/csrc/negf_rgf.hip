@@ -2757,6 +2757,13 @@ void launch_variant(int v, int Np, int slots, hipStream_t st, const TransArgs &a
     else launch_trans<true>(Np, slots, th, st, a);
 }
 
+// Work per slice of one job for the CU shares of an uneven batch: Np^3, with Np counted as
+// at least 160.  Below that the sweep is bound by the latency of the pivot steps, not by
+// arithmetic, so a plain Np^3 share left the small-block groups a few workgroups each, and
+// they finished last (48 devices of N 61-509: 28.5 s; with the floor at 128 / 160 / 224:
+// 17.8 / 15.2 / 15.1 s).
+double share_cost(int Np) { return std::pow((double)std::max(Np, 160), 3); }
+
 // Variant picked per Np by the probe of tune_group, shared by every negf_multi.
 std::mutex g_tune_mu;
 std::map<int, int> g_tuned;
@@ -3052,12 +3059,12 @@ int negf_multi_transmission(negf_multi *m, const double *const *E, const int *nE
         }
         if ((st = job_tables(c, energies, realE, kx[d], ky[d], nk[d], dje[d], djk[d], true, tabs[d]))) return st;
     }
-    // one group per Np; work estimate N_z Np^3 per job gives each group its share of CUs
+    // one group per Np; work estimate N_z share_cost(Np) per job gives each group its share of CUs
     const int ng = (int)m->npv.size();
     std::vector<double> work(ng, 0.0);
     for (int d = 0; d < ndev; ++d) {
         const int g = (int)(std::find(m->npv.begin(), m->npv.end(), m->ctxs[d]->Np) - m->npv.begin());
-        work[g] += (double)nE[d] * nk[d] * m->ctxs[d]->Nz * std::pow((double)m->ctxs[d]->Np, 3);
+        work[g] += (double)nE[d] * nk[d] * m->ctxs[d]->Nz * share_cost(m->ctxs[d]->Np);
     }
     double wsum = 0.0;
     for (double w : work) wsum += w;
@@ -3265,7 +3272,7 @@ int negf_multi_ldos(negf_multi *m, const double *const *E, const int *nE, const 
     std::vector<double> work(ng, 0.0);
     for (int d = 0; d < ndev; ++d) {
         const int g = (int)(std::find(m->npv.begin(), m->npv.end(), m->ctxs[d]->Np) - m->npv.begin());
-        work[g] += (double)nE[d] * nk[d] * m->ctxs[d]->Nz * std::pow((double)m->ctxs[d]->Np, 3);
+        work[g] += (double)nE[d] * nk[d] * m->ctxs[d]->Nz * share_cost(m->ctxs[d]->Np);
     }
     double wsum = 0.0;
     for (double w : work) wsum += w;
