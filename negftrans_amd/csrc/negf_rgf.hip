@@ -55,12 +55,10 @@ static const double PI = 3.141592653589793;
 // R <- R g_m costs n_L N^2 per slice and is exact (no change of the arithmetic that
 // reaches the trace), while the inversion keeps its full N^3 cost.
 #define NEGF_RA 8   // rows of R accumulated per pass of the VALU row product
-// The row product runs on the matrix cores for one device at padded sizes 256..320 (the
-// VALU form's strided loads cost more there than the 16-row MFMA tiles' padding); the
-// VALU form elsewhere and in uneven batches, where the matrix-core form slowed the
-// batch by half (measured).  A batch's T therefore equals the single-device T to the
-// last bit except at those padded sizes, where the two summation orders differ at the
-// 1e-13 level.
+// The row product runs on the matrix cores at padded sizes 256..320 (the VALU form's
+// strided loads cost more there than the 16-row MFMA tiles' padding), on the VALU
+// elsewhere; single-device and uneven-batch kernels make the same choice, so a batch's T
+// equals the single-device one-workgroup T to the last bit.
 #define NEGF_RP_MFMA_LO 256
 #define NEGF_RP_MFMA_HI 320
 
@@ -250,7 +248,7 @@ __global__ __launch_bounds__(MAXT) NEGF_SMALL_WPE(MAXT) void k_transmission(Tran
                 }
                 __syncthreads();
             } else if (m >= 2) {
-                if (!MULTI && Np >= NEGF_RP_MFMA_LO && Np <= NEGF_RP_MFMA_HI)
+                if (Np >= NEGF_RP_MFMA_LO && Np <= NEGF_RP_MFMA_HI)
                     row_product_mfma(Rn, R, nL, ldR, M, s.sigma, Np);   // R <- R g_m
                 else row_product(Rn, R, nL, ldR, M, s.sigma, Np, rs);
                 cd *t = R;
