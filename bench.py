@@ -12,6 +12,7 @@ mixed path (FP32 sweep + FP64 refinement); `value` is the faster one among those
 max |T - T_FP64| <= 1e-6, and "precisions" lists both.  Prints one JSON line.
 """
 import argparse
+import contextlib
 import json
 import os
 import sys
@@ -209,6 +210,10 @@ def main():
         c3 = block_c3.measure(40, (256,))[0]
         c3["note"] = "N_z = 40 of the configuration's 2000; points_per_s_at_Nz2000 scales the measured rate"
         line["other_configs"]["block_c3"] = c3
+        # 48 devices of N 61-509 in one hipGraph against one device at a time
+        from negftrans_amd import uneven
+        with contextlib.redirect_stdout(sys.stderr):
+            line["other_configs"]["uneven_c2"] = uneven.measure()
     print(json.dumps(line), flush=True)
 
 

@@ -20,13 +20,9 @@ import numpy as np
 from . import negf, workloads
 
 
-def main(argv=None):
-    ap = argparse.ArgumentParser(description="uneven batch throughput (negf_multi vs one device at a time)")
-    ap.add_argument("--devices", type=int, default=48)
-    ap.add_argument("--energies", type=int, default=32)
-    ap.add_argument("--out", default="")
-    a = ap.parse_args(argv)
-    wls = workloads.uneven_c2(n_devices=a.devices, n_energy=a.energies)
+def measure(n_devices=48, n_energy=32):
+    """The comparison of the module docstring, as a dict (progress lines on stdout)."""
+    wls = workloads.uneven_c2(n_devices=n_devices, n_energy=n_energy)
     devs, Es = [], []
     for wl in wls:
         NGX, NGY = wl.ngx()
@@ -51,13 +47,26 @@ def main(argv=None):
     pts = sum(len(E) for E in Es)
     diff = max(float(np.max(np.abs(x - y))) for x, y in zip(T_multi, T_seq))
     sizes = sorted(set(-(-d.N // 32) * 32 if d.N <= 512 else d.N for d in devs))
-    res = dict(devices=len(devs), energies_per_device=a.energies, energy_points=pts,
+    res = dict(devices=len(devs), energies_per_device=n_energy, energy_points=pts,
                N_values=sorted(set(int(d.N) for d in devs)),
                N_z_range=[int(min(d.N_z for d in devs)), int(max(d.N_z for d in devs))],
                multi=dict(device_s=dev_multi, wall_s=wall_multi, points_per_s=pts / dev_multi),
                sequential=dict(device_s=dev_seq, wall_s=wall_seq, points_per_s=pts / dev_seq),
                speedup=dev_seq / dev_multi, max_abs_dT=diff,
                variants={str(n): negf.Multi.variant(n) for n in sizes if n <= 512})
+    m.close()
+    for d in devs:
+        d.close()
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="uneven batch throughput (negf_multi vs one device at a time)")
+    ap.add_argument("--devices", type=int, default=48)
+    ap.add_argument("--energies", type=int, default=32)
+    ap.add_argument("--out", default="")
+    a = ap.parse_args(argv)
+    res = measure(a.devices, a.energies)
     line = json.dumps(res)
     print(line, flush=True)
     if a.out:
