@@ -293,13 +293,14 @@ __global__ __launch_bounds__(MAXT) NEGF_SMALL_WPE(MAXT) void k_transmission(Tran
 struct MixedArgs {
     TransArgs a;      // job tables (a.ws unused)
     int nref;         // refinement iterations
-    double tol;       // stop when the last correction of X_{N_z} is below tol |X_{N_z}|
+    double tol;       // stop when the estimated error left in X_{N_z} is below tol |X_{N_z}|
     int ldb;          // columns of X and Y (max n_R over the jobs, <= NEGF_NRH)
     cfl *hist;        // per slot N_z Np^2: g_m, column-permuted like the Gauss-Jordan output
     int *hsig;        // per slot N_z Np: their column permutations
-    cfl *scr;         // per slot 2 * 32 * Np: Gauss-Jordan scratch
     cd *X;            // per slot N_z Np ldb: G(:, 1)(:, open), row-major per slice
     cfl *Y;           // per slot N_z Np ldb: correction sweep vectors
+    int *next;        // job counter (zero at launch): workgroups take jobs in turn, so that
+                      // a job with more refinement iterations holds up only its own slot
 };
 
 struct SolveShared {
@@ -439,7 +440,7 @@ __device__ __forceinline__ void scatter_rhs(cfl *cL, const C *V, const int *sg, 
 
 template <int MAXT>
 __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
-    __shared__ Gje4Shared<32, float> s;
+    __shared__ BgjeShared<float> s;
     __shared__ SolveShared so;
     const TransArgs &a = ma.a;
     const SysView &sv = a.sv;
@@ -449,17 +450,29 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
     const int jj = lane & 15, q = lane >> 4;
     cfl *hist = ma.hist + (size_t)blockIdx.x * Nz * NN;
     int *hsig = ma.hsig + (size_t)blockIdx.x * Nz * Np;
-    cfl *scr = ma.scr + (size_t)blockIdx.x * 2 * 32 * Np;
     cd *X = ma.X + (size_t)blockIdx.x * Nz * Np * ldb;
     cfl *Y = ma.Y + (size_t)blockIdx.x * Nz * Np * ldb;
-    for (int job = blockIdx.x; job < a.njobs; job += gridDim.x) {
-        const int nL = a.nL[job], nR = a.nR[job], off = a.open_off[job];
+    // Jobs are taken in turn from the counter.  The loop has one back edge, through the
+    // fetch; the job body leaves early by returning (a `continue` back to the fetch has
+    // compiled into a loop that skipped the atomic); the job index, the open-channel
+    // counts and the growth flag are wave-uniform (readfirstlane), since the branches on
+    // them enclose barriers.
+    auto fetch = [&]() {
+        if (t == 0) s.job = atomicAdd(ma.next, 1);
+        __syncthreads();
+        const int jb = __builtin_amdgcn_readfirstlane(s.job);
+        __syncthreads();
+        return jb;
+    };
+    auto run_job = [&](const int job) {
+        const int nL = __builtin_amdgcn_readfirstlane(a.nL[job]), nR = __builtin_amdgcn_readfirstlane(a.nR[job]);
+        const int off = __builtin_amdgcn_readfirstlane(a.open_off[job]);
         if (nL == 0 || nR == 0) {             // no open channel: tau = 0 exactly
             if (t == 0) {
                 a.tau[job] = 0.0;
                 a.status[job] = 0;
             }
-            continue;
+            return;
         }
         JobView jv;
         jv.z = a.z[job];
@@ -467,16 +480,27 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
         jv.sigL = a.sigL + (size_t)job * Np;
         jv.sigR = a.sigR + (size_t)job * Np;
         const int *Rc = a.Rcol + off;
-        int bad = 0;
-        // 1. FP32 sweep, every generator kept: g_m = (D_m - g_{m+1})^{-1}
-        for (int m = Nz; m >= 1; --m) {
+        bool grew = false;
+        // 1. FP32 sweep, every generator kept: g_m = (D_m - g_{m+1})^{-1} by the block
+        // Gauss-Jordan (no column map: hsig is the identity)
+        for (int i = t; i < Np; i += blockDim.x) s.sigma[i] = i;
+        __syncthreads();
+        for (int m = Nz; m >= 1 && !grew; --m) {
             cfl *M = hist + (size_t)(m - 1) * NN;
-            build_block<cfl>(M, sv, jv, m - 1, m == Nz ? nullptr : hist + (size_t)m * NN, s.sigma);
+            build_block<cfl>(M, sv, jv, m - 1, m == Nz ? nullptr : hist + (size_t)m * NN, nullptr);
             __syncthreads();
-            if (gje_invert4<32, float>(M, Np, s, scr) && !bad) bad = m;
-            for (int i = t; i < Np; i += blockDim.x) hsig[(size_t)(m - 1) * Np + i] = s.sigma[i];
+            if (bgje_invert4<float>(M, Np, s)) grew = true;
+            for (int i = t; i < Np; i += blockDim.x) hsig[(size_t)(m - 1) * Np + i] = i;
         }
         __syncthreads();
+        if (grew) {                           // growth check failed: the host redoes it in FP64
+            if (t == 0) {
+                a.tau[job] = 0.0;
+                a.status[job] = -1;
+            }
+            __syncthreads();
+            return;
+        }
         // 2. first solve: X_1 = g_1(:, open) = Zg_1(:, sigma_1(open)), X_m = -g_m X_{m-1}
         for (int e = t; e < Np * ldb; e += blockDim.x) {
             const int i = e / ldb, b = e % ldb;
@@ -499,10 +523,12 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
             }
             __syncthreads();
         }
-        // 3. iterative refinement with FP64 residuals, until the correction of X_{N_z} on
-        // the open rows is below ma.tol of |X_{N_z}| there (at most ma.nref iterations;
-        // a job that does not get there reports status -1)
+        // 3. iterative refinement with FP64 residuals, until the error left in X_{N_z} on the
+        // open rows, estimated from the last correction and the observed contraction, is
+        // below ma.tol of |X_{N_z}| there (at most ma.nref iterations; a job that does not get
+        // there reports status -1)
         bool conv = false;
+        double dprev = -1.0;
         for (int it = 0; it < ma.nref && !conv; ++it) {
             for (int m = Nz; m >= 1; --m) {           // y_m = g_m (r_m - y_{m+1})
                 resid_scatter(sv, jv, m, X, Y, ldb, nR, Rc, hsig + (size_t)(m - 1) * Np, so);
@@ -554,7 +580,13 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
             }
             dmx = block_max2(dmx, s.red);
             xmx = block_max2(xmx, s.red);
-            conv = dmx <= ma.tol * xmx;
+            // the error left in X_{N_z} is about |d| rho / (1 - rho), rho = |d| / |d_prev| the
+            // observed contraction: stop when that is below tol |X|.  The first iteration
+            // (no rho yet) stops only on a correction below 1e-3 tol |X|.
+            const double rho = dprev > 0.0 ? dmx / dprev : 1.0;
+            conv = rho < 0.5 ? dmx * rho <= ma.tol * xmx * (1.0 - rho) : dmx <= 1e-3 * ma.tol * xmx;
+            conv = __builtin_amdgcn_readfirstlane((int)conv) != 0;
+            dprev = dmx;
         }
         // 4. trace on X_{N_z} = G_{N_z,1}(:, open)
         double part = 0.0;
@@ -566,10 +598,11 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
         const double tot = block_sum2(part, s.red);
         if (t == 0) {
             a.tau[job] = tot * sv.scale * sv.scale;
-            a.status[job] = bad ? bad : (conv ? 0 : -1);
+            a.status[job] = conv ? 0 : -1;
         }
         __syncthreads();
-    }
+    };
+    for (int job = fetch(); job < a.njobs; job = fetch()) run_job(job);
 }
 
 // D mode (diagonal blocks): backward generators stored per slot, then
@@ -1430,6 +1463,11 @@ __global__ __launch_bounds__(256) void k_lead_decimation(int nj, int N, int Np, 
     iters[o] = done;
 }
 
+// p[0 .. n) = 0 (a kernel rather than a memset, so that it may also sit in a captured graph)
+__global__ __launch_bounds__(64) void k_zero_int(int *p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
 // ------------------------------------------------------------------ host side
 struct negf_ctx {
     int device = 0;
@@ -1445,10 +1483,12 @@ struct negf_ctx {
     // reusable device buffers
     void *ws = nullptr;
     size_t ws_bytes = 0;
-    int mixed_refine = 8;   // refinement iteration limit of NEGF_PREC_MIXED_FP32
-    double mixed_tol = 1e-9; // its stopping tolerance (relative correction of X_{N_z})
+    int mixed_refine = 12;  // refinement iteration limit of NEGF_PREC_MIXED_FP32
+    double mixed_tol = 1e-8; // its stopping tolerance (estimated relative error left in X_{N_z})
     int last_fallbacks = 0; // energies of the last mixed call redone in FP64
     bool diag_one_sided = false; // transmission, D mode, bias window: never the two-sided forms (NEGF_DIAG_ONE_SIDED=1)
+    bool mixed_fallback = true;  // mixed path: recompute unconverged energies in FP64 (NEGF_MIXED_FALLBACK=0 keeps
+                                 // the unconverged mixed values, for measuring the refinement)
 };
 
 namespace {
@@ -1747,6 +1787,7 @@ int negf_ctx_create(negf_ctx **out, int N, int N_z, const int *nx_grid, const in
     if (const char *r = std::getenv("NEGF_MIXED_REFINE")) c->mixed_refine = std::max(0, std::atoi(r));
     if (const char *r = std::getenv("NEGF_MIXED_TOL")) c->mixed_tol = std::max(1e-15, std::atof(r));
     if (const char *r = std::getenv("NEGF_DIAG_ONE_SIDED")) c->diag_one_sided = std::atoi(r) != 0;
+    if (const char *r = std::getenv("NEGF_MIXED_FALLBACK")) c->mixed_fallback = std::atoi(r) != 0;
     c->Lx = Lx;
     c->Ly = Ly;
     c->Lz = Lz;
@@ -1899,9 +1940,9 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
         auto al = [](size_t b) { return (b + 255) / 256 * 256; };
         const size_t Np = c->Np, Nz = c->Nz, ldb = nRmax;
         const size_t b_hist = al(Nz * Np * Np * sizeof(cfl)), b_sig = al(Nz * Np * sizeof(int)),
-                     b_scr = al(2 * 32 * Np * sizeof(cfl)), b_x = al(Nz * Np * ldb * sizeof(cd)),
+                     b_x = al(Nz * Np * ldb * sizeof(cd)),
                      b_y = al(Nz * Np * ldb * sizeof(cfl));
-        const size_t per_slot_m = b_hist + b_sig + b_scr + b_x + b_y;
+        const size_t per_slot_m = b_hist + b_sig + b_x + b_y;
         const void *kfm = shp == SHAPE_640 ? (const void *)k_transmission_mixed<640> : (const void *)k_transmission_mixed<1024>;
         const int slots = slots_for(c, kfm, threads, nj, per_slot_m);
         if ((st = ensure_ws(c, per_slot_m * slots))) return st;
@@ -1929,9 +1970,10 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
         char *base = (char *)c->ws;
         ma.hist = (cfl *)base;
         ma.hsig = (int *)(base + b_hist * slots);
-        ma.scr = (cfl *)(base + (b_hist + b_sig) * slots);
-        ma.X = (cd *)(base + (b_hist + b_sig + b_scr) * slots);
-        ma.Y = (cfl *)(base + (b_hist + b_sig + b_scr + b_x) * slots);
+        ma.X = (cd *)(base + (b_hist + b_sig) * slots);
+        ma.Y = (cfl *)(base + (b_hist + b_sig + b_x) * slots);
+        if ((st = tmp.alloc(&ma.next, 1))) return st;
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, ma.next, 1);
         // each region holds `slots` padded per-slot blocks; the kernel steps through a region
         // with the unpadded per-slot size, which is never larger
         HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -1956,6 +1998,7 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
                 if (mark[i]) redo.push_back(i);
         }
         double extra_s = 0.0;
+        if (!c->mixed_fallback) redo.clear();
         if (!redo.empty()) {
             const int n = (int)redo.size();
             std::vector<double> Er(n), tr(n, 0.0), tj((size_t)n * nk, 0.0);

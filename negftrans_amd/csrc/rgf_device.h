@@ -972,6 +972,238 @@ __device__ __forceinline__ int gje_invert4(typename GjePrec<R>::C *__restrict__ 
 }
 
 // ====================================================================================
+// Block Gauss-Jordan without row interchanges (the sweep of the mixed-precision path).
+// For each block K of 32 rows/columns [k0, k0 + 32):
+//     P = M_KK^{-1}                                  (FP64, by the wave owning the rows)
+//     M_IK <- -M_IK P  (I != K),   M_KK <- P          (matrix cores, precision R)
+//     M_IJ <- M_IJ + M_IK M_KJ,    M_KJ <- P M_KJ      (J != K; the trailing update)
+// This is the in-place Gauss-Jordan inversion of zgetri without pivoting: the 32 scalar
+// elimination steps of a panel become one 32 x 32 inverse and one 32 x 32 product per 32
+// rows.  The trailing update is gje_invert4's (old pivot rows staged in LDS straight from
+// Z, panel values as the B operand).  Stability rests on the diagonal dominance of the RGF
+// blocks D_m - g_{m+1}; the call returns nonzero when P is not finite, |P| exceeds
+// NEGF_BGJE_PMAX or a multiplier |M_IK P| exceeds NEGF_BGJE_MMAX, and the caller then
+// computes the job on the pivoted FP64 path instead.  No column map: inverse = Z.
+// ====================================================================================
+#define NEGF_BGJE_PMAX 1.0e6
+#define NEGF_BGJE_MMAX 1.0e3
+
+template <class R, int MAXNP = NEGF_MAXNP4, int TCH = NEGF_TCH>
+struct BgjeShared {
+    typedef typename GjePrec<R>::C C;
+    static constexpr int PB = 32, MAXW = 2 * MAXNP / 64;
+    static constexpr int CH = TCH, LDT = TCH + 1;   // staged old pivot rows, padded stride
+    static constexpr int LDP = PB + 1;              // diagonal block, padded stride
+    C Tl[PB * LDT];
+    cd Pd[PB * LDP];     // M_KK, inverted in place in FP64: Pd[m * LDP + c]
+    int sigma[MAXNP];    // identity (the consumers read inverse[:, c] = Z[:, sigma[c]])
+    int bad;
+    int job;             // the workgroup's current job (kernels that take jobs in turn)
+    double red[MAXW];
+};
+
+// In-place FP64 Gauss-Jordan inverse (no pivoting) of the 32 x 32 block A[r * lda + c] in
+// LDS by one wave: lane r + 32 h owns row r, columns [16 h, 16 h + 16).  Every lane reads
+// its step operands (pivot row, its row's column J) before any lane of the wave writes,
+// in program order, so the steps need no barrier.  bad: a pivot reciprocal beyond
+// NEGF_BGJE_PMAX (or not finite).
+__device__ __forceinline__ void bgje_diag_inverse(cd *A, const int lda, bool &bad) {
+    const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+    cd *row = A + r * lda + 16 * h;
+#pragma unroll 1
+    for (int J = 0; J < 32; ++J) {
+        const cd *prow = A + J * lda + 16 * h;
+        const cd d = A[J * lda + J];
+        const cd m = A[r * lda + J];
+        const cd inv = crecip_fast(d);
+        if (!(cabs1(inv) <= NEGF_BGJE_PMAX)) bad = true;
+        const bool piv = (r == J);
+        const cd f = piv ? cmk(-inv.x, -inv.y) : cmul(m, inv);
+        // eight columns at a time: a chunk's stores touch only its own columns, so the
+        // next chunk's loads still see the step's old values
+#pragma unroll
+        for (int c0 = 0; c0 < 16; c0 += 8) {
+            cd pv[8], x[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                pv[c] = prow[c0 + c];
+                x[c] = row[c0 + c];
+            }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const cd n = cfms(f, pv[c], piv ? cmk(0.0, 0.0) : x[c]);
+                row[c0 + c] = (16 * h + c0 + c == J) ? cmk(-f.x, -f.y) : n;
+            }
+        }
+    }
+}
+
+// In-place inversion of Z (Np x Np, Np % 32 == 0, blockDim.x == 2 Np); on return
+// inverse = Z and s.sigma is the identity.  Returns 0, or 1 if the growth check failed.
+template <class R, class SH>
+__device__ __forceinline__ int bgje_invert4(typename GjePrec<R>::C *__restrict__ Z, const int Np, SH &s) {
+    typedef GjePrec<R> G;
+    typedef typename G::C C;
+    typedef typename G::Acc Acc;
+    constexpr int PB = 32, KK = 8, LDP = SH::LDP;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int j = lane & 15, k = lane >> 4;
+    const int R0 = w << 5;
+    const size_t ld = (size_t)Np;
+    for (int i = t; i < Np; i += blockDim.x) s.sigma[i] = i;
+    if (t == 0) s.bad = 0;
+    bool bad = false;
+    for (int k0 = 0; k0 < Np; k0 += PB) {
+        C P[2][KK];
+#pragma unroll
+        for (int q = 0; q < KK; ++q) {
+            const C *zc = Z + (R0 + j) + (size_t)(k0 + 4 * q + k) * ld;
+            P[0][q] = zc[0];
+            P[1][q] = zc[16];
+        }
+        const bool owner = (R0 == k0);
+        if (owner) {
+            // P = M_KK^{-1} in FP64; the owner's rows are the pivot rows: they take P
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+                for (int q = 0; q < KK; ++q) s.Pd[(16 * tt + j) * LDP + 4 * q + k] = to_cd(P[tt][q]);
+            bgje_diag_inverse(s.Pd, LDP, bad);
+            double pm = 0.0;
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+                for (int q = 0; q < KK; ++q) {
+                    const cd x = s.Pd[(16 * tt + j) * LDP + 4 * q + k];
+                    pm = fmax(pm, cabs1(x));
+                    P[tt][q] = from_cd<C>(x);
+                }
+            if (!(pm <= NEGF_BGJE_PMAX)) bad = true;
+        }
+        __syncthreads();
+        if (!owner) {
+            // panel values -M_IK P: D = A B on the matrix cores with A[i][kk] = -P[4 q + kk][c(i)],
+            // c(i) = 16 h + 4 (i % 4) + i / 4, and B[kk][jj] = M_IK[16 t + jj][4 q + kk] = P[t][q];
+            // D[i][jj] lands in lane jj + 16 (i / 4), register i % 4, i.e. panel column
+            // 16 h + 4 r + k of lane j + 16 k, register r: the panel layout again (q = 4 h + r).
+            // (FP64 accumulators hold row i in lane group i % 4, register i / 4: c(i) = 16 h + i)
+            const int ci = G::CQ == 1 ? (lane & 15) : (((lane & 3) << 2) | ((lane >> 2) & 3));
+            C Pv[2][KK];
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    Acc ar, ai;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        ar[r] = R(0);
+                        ai[r] = R(0);
+                    }
+#pragma unroll
+                    for (int q = 0; q < KK; ++q) {
+                        const cd pd = s.Pd[(4 * q + k) * LDP + 16 * h + ci];
+                        const C a = from_cd<C>(cmk(-pd.x, -pd.y));
+                        const C b = P[tt][q];
+                        ar = G::mfma(a.x, b.x, ar);
+                        ar = G::mfma(-a.y, b.y, ar);
+                        ai = G::mfma(a.x, b.y, ai);
+                        ai = G::mfma(a.y, b.x, ai);
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) Pv[tt][4 * h + r] = cmk(ar[r], ai[r]);
+                }
+            R vmax = R(0);
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+                for (int q = 0; q < KK; ++q) {
+                    P[tt][q] = Pv[tt][q];
+                    const R v = cabs1(Pv[tt][q]);
+                    vmax = v > vmax ? v : vmax;
+                }
+            if (!(vmax <= R(NEGF_BGJE_MMAX))) bad = true;
+        }
+        // panel columns take their final values
+#pragma unroll
+        for (int q = 0; q < KK; ++q) {
+            C *zc = Z + (R0 + j) + (size_t)(k0 + 4 * q + k) * ld;
+            zc[0] = P[0][q];
+            zc[16] = P[1][q];
+        }
+        // trailing update on the matrix cores, one chunk of SH::CH columns at a time: the
+        // old pivot rows M_KJ are staged from Z (their new values are written only by the
+        // chunk's own update, after the staging barrier); the pivot rows start from zero
+        const bool zb = owner;
+        const __amdgpu_buffer_rsrc_t zr = negf_rsrc(Z, (unsigned)(Np * Np) * G::ESZ);
+        const unsigned cstr = (unsigned)G::CR * (unsigned)Np * G::ESZ;
+        auto next_tile = [&](int ct) {
+            const int n = (ct + 16 == k0) ? k0 + PB : ct + 16;
+            return n < Np ? n : ct;
+        };
+        auto voff = [&](int ct) { return (unsigned)((R0 + j) + (ct + G::CQ * k) * Np) * G::ESZ; };
+        C nb0[4], nb1[4];
+        auto load_base = [&](int ct) {
+            const unsigned vo = voff(ct);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                nb0[q] = G::load(zr, vo, q * cstr);
+                nb1[q] = G::load(zr, vo + 16u * G::ESZ, q * cstr);
+            }
+        };
+        load_base(k0 == 0 ? PB : 0);
+        for (int c0 = 0; c0 < Np; c0 += SH::CH) {
+            const int c1 = min(c0 + SH::CH, Np);
+            if (c0 >= k0 && c1 <= k0 + PB) continue;       // chunk is all panel
+            for (int e = t; e < (c1 - c0) * PB; e += blockDim.x) {
+                const int pp = e % PB, cl = e / PB;
+                const int col = c0 + cl;
+                if (!(col >= k0 && col < k0 + PB)) s.Tl[pp * SH::LDT + cl] = Z[(k0 + pp) + (size_t)col * ld];
+            }
+            __syncthreads();
+            for (int ct = c0; ct < c1; ct += 16) {
+                if (ct >= k0 && ct < k0 + PB) continue;
+                Acc ar0, ai0, ar1, ai1;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    ar0[q] = zb ? R(0) : nb0[q].x;
+                    ai0[q] = zb ? R(0) : nb0[q].y;
+                    ar1[q] = zb ? R(0) : nb1[q].x;
+                    ai1[q] = zb ? R(0) : nb1[q].y;
+                }
+                load_base(next_tile(ct));
+                __builtin_amdgcn_sched_barrier(0);          // keep the prefetch ahead of the MFMAs
+                int lx = threadIdx.x & 63;
+                asm volatile("" : "+v"(lx));                 // recompute, do not keep live
+                const C *tl = s.Tl + (lx >> 4) * SH::LDT + (ct - c0 + (lx & 15));
+#pragma unroll
+                for (int q = 0; q < KK; ++q) {
+                    const C tv = tl[4 * q * SH::LDT];
+                    const R ntim = -tv.y;
+                    ar0 = G::mfma(tv.x, P[0][q].x, ar0);
+                    ar1 = G::mfma(tv.x, P[1][q].x, ar1);
+                    ai0 = G::mfma(tv.y, P[0][q].x, ai0);
+                    ai1 = G::mfma(tv.y, P[1][q].x, ai1);
+                    ar0 = G::mfma(ntim, P[0][q].y, ar0);
+                    ar1 = G::mfma(ntim, P[1][q].y, ar1);
+                    ai0 = G::mfma(tv.x, P[0][q].y, ai0);
+                    ai1 = G::mfma(tv.x, P[1][q].y, ai1);
+                }
+                const unsigned vo = voff(ct);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    G::store(cmk(ar0[q], ai0[q]), zr, vo, q * cstr);
+                    G::store(cmk(ar1[q], ai1[q]), zr, vo + 16u * G::ESZ, q * cstr);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (__ballot(bad) && lane == 0) atomicOr(&s.bad, 1);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(s.bad);   // uniform: callers branch on it around barriers
+}
+
+// ====================================================================================
 // The same matrix-core Gauss-Jordan for 512 < Np <= 1024: every wave owns NT = 4 row
 // tiles (64 rows; ceil(Np / 64) <= 16 waves, so at most 1024 threads) and the panel is
 // PB = 16 columns wide, which keeps the panel registers at NT * PB / 4 = 16 complex values
