@@ -136,6 +136,20 @@ int negf_inverse_batch(int N, int count, const double *A, double *inv);
 int negf_matrix_product_batch(int N, int count, const double *A, const double *B, double *C);
 
 /*
+ * negf_inverse_batch without any allocation.  It has no counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  A, inv (count N x N complex128 each) and
+ * status (count ints: 0, or the 1-based column of the first exactly-zero pivot) are
+ * DEVICE pointers owned by the caller; the work areas come from a 64 MiB array the
+ * library holds on each GPU, so count * (Np^2 + 2 PB Np) * 16 bytes must fit in it
+ * (Np = N padded to 32, PB = 32; above Np = 512, padded to 16 and PB = 16) -- larger
+ * batches are rejected with an error.  The work is queued on `stream` (a hipStream_t;
+ * null = the default stream) and not waited for.  Calls that overlap on one GPU race on
+ * the shared array.  Returns 0 or a negative error code.
+ */
+int negf_inverse_batch_noalloc(int N, int count, const double *A, double *inv, int *status, void *stream);
+
+/*
  * Slice blocks on the GPU, for callers that want them materialised:
  *   negf_hamiltonian : H(:, :, k) of Hamiltonian_construction (src/grid.f90:236-262)
  *   negf_e_minus_h   : D_k of E_minus_H_construction (src/grid.f90:264-333) at complex

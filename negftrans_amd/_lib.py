@@ -39,6 +39,7 @@ SIGNATURES = {
     "negf_nonlocal_vhat_from_real": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int,
                                                     _dp]),
     "negf_inverse_batch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "negf_inverse_batch_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "negf_matrix_product_batch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
     "negf_last_kernel_seconds": (ctypes.c_double, [_vp]),
     "negf_last_fallbacks": (ctypes.c_int, [_vp]),

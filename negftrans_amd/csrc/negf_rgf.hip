@@ -2565,6 +2565,63 @@ int negf_inverse_batch(int N, int count, const double *A, double *inv) {
     return 0;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------------ allocation-free inverse
+// negf_inverse_batch_noalloc takes the Gauss-Jordan work areas of k_inverse_batch from
+// `scratch`, a file-scope array in the device memory of every GPU the library runs on
+// (NEGF_NOALLOC_SCRATCH_BYTES, held until the process ends).  In host code the name
+// `scratch` is a host-side stand-in, not a device address: every host use goes through
+// the address hipGetSymbolAddress returned for the current device, kept per device.
+// k_inverse_batch writes each work area in full (the padded matrix, then the pivot-row
+// staging) before reading it, so nothing needs clearing between calls; the array keeps
+// the last call's contents, and calls that overlap on one GPU race on it.
+#define NEGF_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+static __device__ __align__(256) unsigned char scratch[NEGF_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_dev[64];   // per device: the address of `scratch`, or null before the first call
+
+int scratch_address(void **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(-7, "device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(-100 - (int)e, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int negf_inverse_batch_noalloc(int N, int count, const double *A, double *inv, int *status, void *stream) {
+    if (N < 1 || count < 0 || (count > 0 && (!A || !inv || !status)))
+        return fail(-2, "bad arguments to negf_inverse_batch_noalloc");
+    const int Np = padded_size(N);
+    if (Np > NEGF_MAXNP) return fail(-5, "N too large (N <= 1024)");
+    const size_t per = ((size_t)Np * Np + 2 * (size_t)panel_width(Np) * Np) * sizeof(cd);
+    if ((size_t)count * per > NEGF_NOALLOC_SCRATCH_BYTES)
+        return fail(-6, "negf_inverse_batch_noalloc: " + std::to_string(count) + " matrices of N = " +
+                            std::to_string(N) + " need " + std::to_string((size_t)count * per) +
+                            " bytes of work area; the scratch holds " + std::to_string(NEGF_NOALLOC_SCRATCH_BYTES) +
+                            " (at most " + std::to_string(NEGF_NOALLOC_SCRATCH_BYTES / per) + " per call)");
+    if (count == 0) return 0;
+    void *w = nullptr;
+    if (int st = scratch_address(&w)) return st;
+    launch_inverse(Np, count, (hipStream_t)stream, N, reinterpret_cast<const cd *>(A), nullptr,
+                   static_cast<cd *>(w), reinterpret_cast<cd *>(inv), status);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int negf_matrix_product_batch(int N, int count, const double *A, const double *B, double *C) {
     if (N < 1 || count < 0 || (count > 0 && (!A || !B || !C))) return fail(-2, "bad arguments to negf_matrix_product_batch");
     if (count == 0) return 0;
