@@ -50,3 +50,25 @@ def test_adaptive_grid_accuracy_and_gain(trained):
     assert s["fallback_rate"] == 0.0
     assert s["within_1e_4"] == 1.0 and s["points_max"] <= 2048 // 8
     assert s["points_mean"] <= 0.95 * s["no_network_points_mean"]
+
+
+def test_adaptive_grid_c4_size():
+    """The adaptive grid at the configuration's own size (N = 113, N_z = 400, 512 bins,
+    dense judge of 8192 RGF energies).  The network is trained on 64 devices only (the
+    configuration's 20k would take hours of RGF), so the bar is the method's guarantee,
+    not the network's gain: every held-out device within 1e-4 of the dense integrated
+    conductance, with at most 1/8 of the dense points, whether or not it fell back."""
+    from negftrans_amd import adaptive as A, negf
+    from negftrans_amd.mlp import TransmissionMLP
+    if negf.device_count() < 1:
+        pytest.fail("no GPU visible to libnegf_gpu.so")
+    kw = dict(N_z=400, R2=36.5)
+    X, T = A.dataset(range(2000, 2064), 512, **kw)
+    assert X.shape == (64, 2048) and T.shape == (64, 512)
+    m = TransmissionMLP(X.shape[1], 512, seed=0)
+    hist = m.fit(X, T, epochs=150, batch=16, lr=1e-3)
+    assert hist[-1] < hist[0]
+    rows = A.judge(m, range(910000, 910004), 512, 8192, 1e-4, 48, **kw)
+    s = A.summary(rows, 8192)
+    print(s)
+    assert s["within_1e_4"] == 1.0 and s["points_max"] <= 8192 // 8

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import negf_oracle as O
-from tests.test_gpu_parity import device_for, oracle_system, small_system
+from tests.test_gpu_parity import device_for, oracle_system, small_system  # noqa: F401
 
 pytestmark = pytest.mark.gpu
 
@@ -71,6 +71,27 @@ def test_mixed_disorder_c1_gate(ng):
     dT = np.max(np.abs(mix - f64))
     print(f"max |dT| mixed vs FP64 over {len(E)} energies: {dT:.3e}")
     assert dT <= 1e-6
+
+
+def test_mixed_disorder_c1_256_energies_vs_oracle(ng):
+    """The mixed path as the benchmark runs it (N = 317, N_z = 1000, 256 energies in one
+    call, workgroups taking jobs in turn): every energy within the 1e-6 gate of the FP64
+    GPU path, and two energies within 1e-7 of the oracle (the refinement's target: an
+    estimated relative error of 1e-8 left in G_{N_z,1})."""
+    from negftrans_amd import workloads
+    wl = workloads.disorder_c1()
+    NGX, NGY = wl.ngx()
+    vrec = ng.local_potential_real_to_planewave(wl.V_real_eV / O.HARTREE, NGX, NGY)
+    E = wl.energies_hartree()[(np.arange(256) * 16 + 5) % 4096]
+    dev = ng.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY, flags=ng.PREC_MIXED_FP32)
+    mix = dev.transmission(E)
+    assert dev.last_status == 0
+    f64 = ng.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY).transmission(E)
+    assert np.max(np.abs(mix - f64)) <= 1e-6
+    s = oracle_system(wl, np.asarray(vrec), NGX, NGY)
+    for i in (40, 201):
+        ref = O.transmission_tau(s, E[i], 0, faithful_solver=False)
+        assert abs(mix[i] - ref) <= 1e-7
 
 
 def test_mixed_rejects_more_than_16_open_channels(ng):

@@ -199,3 +199,65 @@ def test_multi_uneven_block_sizes_vs_single(ng, variant):
     for d, e, g, slc in zip(devs, E2, L, sl):
         ref = d.ldos_matrix(e)[:, :, slc, :]
         assert np.max(np.abs(g - ref)) <= 1e-12 * np.abs(ref).max()
+
+
+def test_uneven_c2_workload(ng, variant):
+    """The uneven-batch configuration itself: the 48 devices of workloads.uneven_c2 (N in
+    {61, 113, 197, 317, 509}, N_z from 228 to 1451, seeded disorder, m* = 0.067), all in
+    one graph.  64 of the configuration's 256 energies per device (a strided quarter of
+    its 0-30 meV grid), to keep the test near a minute; the benchmark's uneven leg times
+    the full batch.  Checks, each in its assertion:
+    - 0 <= T <= number of open channels on every device and energy;
+    - the N = 61 device (N_z = 228) and the N = 509 device with the shortest wire
+      (N_z = 305) against the oracle at relative 1e-10 on two energies each;
+    - LDOS blocks at 8 slices per device (two energies): the spectral part
+      (L - L^dagger) / 2i of each block is positive semidefinite (L = -G / pi, retarded),
+      and two devices equal their own single-device negf_ldos_batch at those slices."""
+    variant(None)
+    from negftrans_amd import workloads
+    wls = workloads.uneven_c2()
+    devs, Es, vrecs = [], [], []
+    for wl in wls:
+        NGX, NGY = wl.ngx()
+        vrec = ng.local_potential_real_to_planewave(wl.V_real_eV / O.HARTREE, NGX, NGY)
+        vrecs.append(np.asarray(vrec))
+        devs.append(ng.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY))
+        Es.append(wl.energies_hartree()[1::4])
+    assert sorted({d.N for d in devs}) == [61, 113, 197, 317, 509] and len(devs) == 48
+    m = ng.Multi(devs)
+    T = m.transmission(Es)
+    assert m.last_status == 0
+    for wl, d, E, t in zip(wls, devs, Es, T):
+        nx, ny = wl.basis()
+        a = wl.atomic()
+        kin = O.kinetic(nx, ny, a.LX, a.LY, 0.0, 0.0)
+        n_open = np.array([np.sum(2 * e - kin > 0) for e in E])
+        assert np.all(t >= -1e-12) and np.all(t <= n_open + 1e-6), d.N
+    picks = [i for i, wl in enumerate(wls) if wl.extra["N"] == 61][:1]
+    picks += [min((i for i, wl in enumerate(wls) if wl.extra["N"] == 509), key=lambda i: wls[i].extra["N_z"])]
+    for i in picks:
+        wl = wls[i]
+        NGX, NGY = wl.ngx()
+        a = O.atomic_from_input(**{k: v for k, v in wl.inp.items()
+                                   if k in ("LX", "LY", "LZ", "ENCUT", "ETA", "TRANSMISSION_GRID")})
+        nx, ny = O.planewave_basis(a.ENCUT, a.LX, a.LY)
+        s = O.System(a, nx, ny, vrecs[i], NGX, NGY)
+        for j in (20, 51):
+            ref = O.transmission_tau(s, Es[i][j], 0, faithful_solver=False)
+            assert T[i][j] == pytest.approx(ref, rel=RTOL, abs=1e-12), (wl.extra, j)
+    sl = [np.linspace(0, d.N_z - 1, 8).round().astype(int) for d in devs]
+    E2 = [e[[30, 50]] for e in Es]
+    L = m.ldos(E2, sl)
+    assert m.last_status == 0
+    for d, g in zip(devs, L):
+        for q in range(g.shape[2]):
+            for e in range(g.shape[3]):
+                B = g[:, :, q, e]
+                w = np.linalg.eigvalsh((B - B.conj().T) / 2j)
+                assert w.min() >= -1e-9 * max(1.0, np.abs(w).max()), (d.N, q, e)
+    for i in picks:
+        ref = devs[i].ldos_matrix(E2[i])[:, :, sl[i], :]
+        assert np.max(np.abs(L[i] - ref)) <= 1e-12 * np.abs(ref).max()
+    m.close()
+    for d in devs:
+        d.close()

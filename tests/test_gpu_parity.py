@@ -220,6 +220,44 @@ def test_transmission_disorder_c1_full_size(ng):
         assert got[i] == pytest.approx(ref, rel=RTOL, abs=1e-12)
 
 
+def test_transmission_one_sided_padded_320(ng, monkeypatch):
+    """The form the benchmark times: one workgroup per energy (k_transmission), whose
+    open-row product runs on the matrix cores at padded sizes 256-320.  N = 317 (padded
+    320) on a short wire (N_z = 6) against the oracle at relative 1e-10, energies with up
+    to ~70 open channels."""
+    s, _ = small_system(N_z=6, seed=11, R2=100.5)
+    assert len(s.nx) == 317
+    E = np.array([0.3, 1.1, 2.7])
+    ref = O.transmission(s, E)
+    monkeypatch.setenv("NEGF_DIAG_ONE_SIDED", "1")
+    dev = device_for(ng, s)
+    monkeypatch.delenv("NEGF_DIAG_ONE_SIDED")
+    got = dev.transmission(E)
+    assert dev.last_status == 0
+    np.testing.assert_allclose(got, ref, rtol=RTOL, atol=1e-12)
+    assert ref.max() > 10.0
+
+
+def test_transmission_disorder_c1_one_sided(ng, monkeypatch):
+    """The benchmark's kernel at the metric configuration (N = 317, N_z = 1000), forced to
+    one workgroup per energy: two energies against the oracle's O(N^2)-state
+    restatement (same generators) at relative 1e-10."""
+    from negftrans_amd import workloads
+    wl = workloads.disorder_c1()
+    NGX, NGY = wl.ngx()
+    vrec = ng.local_potential_real_to_planewave(wl.V_real_eV / O.HARTREE, NGX, NGY)
+    monkeypatch.setenv("NEGF_DIAG_ONE_SIDED", "1")
+    dev = ng.NegfDevice(*wl.basis(), wl.atomic(), vrec, NGX, NGY)
+    monkeypatch.delenv("NEGF_DIAG_ONE_SIDED")
+    E = wl.energies_hartree()[[1234, 3777]]
+    got = dev.transmission(E)
+    assert dev.last_status == 0
+    s = oracle_system(wl, np.asarray(vrec), NGX, NGY)
+    for i in range(2):
+        ref = O.transmission_tau(s, E[i], 0, faithful_solver=False)
+        assert got[i] == pytest.approx(ref, rel=RTOL, abs=1e-12)
+
+
 def test_density_c1_full_size(ng):
     """D mode at the metric configuration's size (N = 317, N_z = 1000): one contour node
     of the density (z in the upper half plane, weight 1) against the oracle's D-mode

@@ -504,3 +504,74 @@ def test_nonlocal_of_a_local_potential():
                 for qq in range(-R, R + 1):
                     want = loc[p - pp + 2 * R, q - qq + 2 * R] / (Nx * Ny)
                     assert abs(got[p + R, q + R, pp + R, qq + R] - want) < 1e-14
+
+
+# ------------------------------------------------------------------ output units
+def test_density_real_charge_sum_rule():
+    """density_real (src/main.f90:341-349) against the number of electrons: the density
+    summed over the N_x x N_y x N_z grid times the cell volume (LX/N_x)(LY/N_y)(LZ/N_z)
+    equals Im Tr of the slice's density matrix, slice by slice.  A wrong N_z factor, a
+    normalised transform or a missing 1/(LX LY LZ) fails it; so does a transform that
+    does not reduce to the trace (every basis difference n_i - n_j != 0 integrates to 0)."""
+    L = 7.0
+    ENCUT = 9.5 * (2 * np.pi / L) ** 2 / 2
+    nx, ny = O.planewave_basis(ENCUT, L, L)
+    N, N_z, N_x, N_y = len(nx), 4, 14, 10
+    LXa, LYa, LZa = 23.0, 17.0, 31.0                 # Angstrom, unequal on purpose
+    r = rng(12)
+    DM = r.standard_normal((N_z, N, N)) + 1j * r.standard_normal((N_z, N, N))
+    dens = O.density_real(DM, nx, ny, N_x, N_y, LXa, LYa, LZa)
+    dV = (LXa / N_x) * (LYa / N_y) * (LZa / N_z)
+    for k in range(N_z):
+        assert dens[:, :, k].sum() * dV == pytest.approx(np.trace(DM[k]).imag, rel=1e-12, abs=1e-12)
+    # one diagonal element: a uniform density Im(M_ii) / slice volume
+    M = np.zeros((1, N, N), dtype=np.complex128)
+    M[0, 3, 3] = 0.25j
+    d1 = O.density_real(M, nx, ny, N_x, N_y, LXa, LYa, LZa)
+    np.testing.assert_allclose(d1, 0.25 / (LXa * LYa * LZa), rtol=1e-12)
+
+
+def test_ldos_units_against_lattice_dos():
+    """ldos_matrix + ldos_reduce (src/negf.f90:176-236, src/main.f90:510-525,
+    src/tools.f90:18-40) against the textbook density of states of the device's own
+    lattice.  With V = 0 and the decimated (lattice) leads the wire is an infinite chain
+    per plane wave, tridiag(1, d_i, 1) with d_i = 2 Delta^2 (E - |G_i|^2 / 2) - 2, whose
+    site Green's function is -i / sqrt(4 - d_i^2) in the band; G = 2 Delta^2 A^-1 makes
+    the DOS per slice sum_i 2 Delta^2 / (pi sqrt(4 - d_i^2)) per Hartree.  Reduced with
+    LDOS_GRID = (-1, -1, 0) (integrate x and y, keep z) the reference's output is that
+    per unit length (/ Delta in Angstrom) and per eV (/ hartree); summed over all three
+    axes (-1, -1, -1) it is the DOS of the whole wire per eV."""
+    import dataclasses
+    s, _ = tiny_system(N_z=7, seed=5, eta=1e-8)
+    s.Vrec[:] = 0
+    sd = dataclasses.replace(s, lead="decimation")
+    a = s.atomic
+    delta = a.LZ / s.N_z                               # bohr
+    half_g2 = 0.5 * O.kinetic(s.nx, s.ny, a.LX, a.LY, 0.0, 0.0)     # |G_i|^2 / 2
+    N_x = N_y = 8
+    LXa, LYa, LZa = a.LX * O.A_0, a.LY * O.A_0, a.LZ * O.A_0
+    for E in (0.35, 1.3, 2.2):
+        d = 2 * delta ** 2 * (E - half_g2) - 2
+        inband = np.abs(d) < 2
+        assert inband.any()
+        dos_slice = np.sum(2 * delta ** 2 / (np.pi * np.sqrt(4 - d[inband] ** 2)))   # per Hartree
+        LM = O.ldos_matrix(sd, E)
+        z = O.ldos_reduce(LM, s.nx, s.ny, N_x, N_y, LXa, LYa, LZa, (-1, -1, 0))[0, 0, :]
+        np.testing.assert_allclose(z, dos_slice / (delta * O.A_0) / O.HARTREE, rtol=1e-5)
+        tot = O.ldos_reduce(LM, s.nx, s.ny, N_x, N_y, LXa, LYa, LZa, (-1, -1, -1))[0, 0, 0]
+        assert tot == pytest.approx(s.N_z * dos_slice / O.HARTREE, rel=1e-5)
+
+
+@pytest.mark.parametrize("L,R2", [(400.0, 19.0), (400.0, 100.5), (400.0, 256.5), (123.0, 7.3), (55.5, 40.2)])
+def test_ngx_window_holds_every_basis_difference(L, R2):
+    """The NGX/NGY auto rule (src/main.f90:94-100) against what the window is for:
+    V_reciprocal_all(-NGX:NGX, -NGY:NGY) must hold every plane-wave difference
+    n_i - n_j that Hamiltonian_construction reads (src/grid.f90:252), i.e.
+    NGX >= 2 max |n_x| over the basis, found here by enumerating the basis."""
+    L_ang = L
+    Lb = L_ang / O.A_0
+    ENCUT_eV = R2 * (2 * np.pi / Lb) ** 2 / 2 * O.HARTREE
+    nx, ny = O.planewave_basis(ENCUT_eV / O.HARTREE, Lb, Lb)
+    NGX, NGY = O.ngx_auto(0, 0, L_ang, L_ang, ENCUT_eV)
+    assert NGX >= 2 * np.max(np.abs(nx)) and NGY >= 2 * np.max(np.abs(ny))
+    assert NGX <= 2 * np.max(np.abs(nx)) + 3          # and no larger than the rounding needs
