@@ -293,7 +293,7 @@ __global__ __launch_bounds__(MAXT) NEGF_SMALL_WPE(MAXT) void k_transmission(Tran
 struct MixedArgs {
     TransArgs a;      // job tables (a.ws unused)
     int nref;         // refinement iterations
-    double tol;       // stop when the estimated error left in X_{N_z} is below tol |X_{N_z}|
+    double tol;       // stop when the estimated error of the job's T is below tol
     int ldb;          // columns of X and Y (max n_R over the jobs, <= NEGF_NRH)
     cfl *hist;        // per slot N_z Np^2: g_m, column-permuted like the Gauss-Jordan output
     int *hsig;        // per slot N_z Np: their column permutations
@@ -301,6 +301,7 @@ struct MixedArgs {
     cfl *Y;           // per slot N_z Np ldb: correction sweep vectors
     int *next;        // job counter (zero at launch): workgroups take jobs in turn, so that
                       // a job with more refinement iterations holds up only its own slot
+    const int *order; // the order in which the jobs are taken (likely slow ones first)
 };
 
 struct SolveShared {
@@ -313,6 +314,9 @@ struct SolveShared {
 // Wave w computes rows [32 w, 32 w + 32) as two 16-row tiles with A = Zg tile
 // (A[i][k] at lane i + 16 k) and B = cL (B[k][b] at lane b + 16 k); lane b + 16 q,
 // register r of tile t holds out(32 w + 16 t + 4 q + r, b).
+// The generator is streamed from HBM in groups of 8 K-steps (32 columns) with the next
+// group's loads issued before the current group's 64 MFMAs (two register buffers), so the
+// load latency hides behind the matrix cores; Np is a multiple of 32.
 __device__ __forceinline__ void gen_matvec(const cfl *Zg, const cfl *cL, int Np, negf_f4 (&ore)[2],
                                            negf_f4 (&oim)[2]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -323,20 +327,43 @@ __device__ __forceinline__ void gen_matvec(const cfl *Zg, const cfl *cL, int Np,
         ore[tt] = negf_f4{0.f, 0.f, 0.f, 0.f};
         oim[tt] = negf_f4{0.f, 0.f, 0.f, 0.f};
     }
-#pragma unroll 4
-    for (int K = 0; K < Np; K += 4) {
-        const unsigned so = (unsigned)K * (unsigned)Np * 8u;
-        const cfl a0 = GjePrec<float>::load(zr, vo, so), a1 = GjePrec<float>::load(zr, vo + 128u, so);
-        const cfl b = cL[(K + k) * NEGF_NRH + i];
-        const float nby = -b.y;
-        ore[0] = GjePrec<float>::mfma(a0.x, b.x, ore[0]);
-        ore[1] = GjePrec<float>::mfma(a1.x, b.x, ore[1]);
-        oim[0] = GjePrec<float>::mfma(a0.x, b.y, oim[0]);
-        oim[1] = GjePrec<float>::mfma(a1.x, b.y, oim[1]);
-        ore[0] = GjePrec<float>::mfma(a0.y, nby, ore[0]);
-        ore[1] = GjePrec<float>::mfma(a1.y, nby, ore[1]);
-        oim[0] = GjePrec<float>::mfma(a0.y, b.x, oim[0]);
-        oim[1] = GjePrec<float>::mfma(a1.y, b.x, oim[1]);
+    constexpr int GK = 8;                       // K-steps (of 4 columns) per group
+    const unsigned gstr = 4u * (unsigned)Np * 8u;
+    auto load = [&](cfl (&A)[GK][2], int g) {
+#pragma unroll
+        for (int s = 0; s < GK; ++s) {
+            const unsigned so = (unsigned)(g * GK + s) * gstr;
+            A[s][0] = GjePrec<float>::load(zr, vo, so);
+            A[s][1] = GjePrec<float>::load(zr, vo + 128u, so);
+        }
+    };
+    auto compute = [&](const cfl (&A)[GK][2], int g) {
+#pragma unroll
+        for (int s = 0; s < GK; ++s) {
+            const cfl b = cL[((g * GK + s) * 4 + k) * NEGF_NRH + i];
+            const float nby = -b.y;
+            ore[0] = GjePrec<float>::mfma(A[s][0].x, b.x, ore[0]);
+            ore[1] = GjePrec<float>::mfma(A[s][1].x, b.x, ore[1]);
+            oim[0] = GjePrec<float>::mfma(A[s][0].x, b.y, oim[0]);
+            oim[1] = GjePrec<float>::mfma(A[s][1].x, b.y, oim[1]);
+            ore[0] = GjePrec<float>::mfma(A[s][0].y, nby, ore[0]);
+            ore[1] = GjePrec<float>::mfma(A[s][1].y, nby, ore[1]);
+            oim[0] = GjePrec<float>::mfma(A[s][0].y, b.x, oim[0]);
+            oim[1] = GjePrec<float>::mfma(A[s][1].y, b.x, oim[1]);
+        }
+    };
+    const int ng = Np / (4 * GK);
+    cfl A0[GK][2], A1[GK][2];
+    load(A0, 0);
+    for (int g = 0; g < ng; g += 2) {
+        if (g + 1 < ng) load(A1, g + 1);
+        __builtin_amdgcn_sched_barrier(0);      // keep the prefetch ahead of the MFMAs
+        compute(A0, g);
+        if (g + 1 < ng) {
+            if (g + 2 < ng) load(A0, g + 2);
+            __builtin_amdgcn_sched_barrier(0);
+            compute(A1, g + 1);
+        }
     }
 }
 
@@ -440,7 +467,7 @@ __device__ __forceinline__ void scatter_rhs(cfl *cL, const C *V, const int *sg, 
 
 template <int MAXT>
 __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
-    __shared__ BgjeShared<float> s;
+    __shared__ BgjeShared<> s;
     __shared__ SolveShared so;
     const TransArgs &a = ma.a;
     const SysView &sv = a.sv;
@@ -489,7 +516,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
             cfl *M = hist + (size_t)(m - 1) * NN;
             build_block<cfl>(M, sv, jv, m - 1, m == Nz ? nullptr : hist + (size_t)m * NN, nullptr);
             __syncthreads();
-            if (bgje_invert4<float>(M, Np, s)) grew = true;
+            if (bgje_invert64(M, Np, s)) grew = true;
             for (int i = t; i < Np; i += blockDim.x) hsig[(size_t)(m - 1) * Np + i] = i;
         }
         __syncthreads();
@@ -568,25 +595,36 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
                 }
                 __syncthreads();
             }
-            // convergence: max |d_{N_z}| against max |X_{N_z}| over the open rows
-            double dmx = 0.0, xmx = 0.0;
+            // convergence, on the transmission itself: with rho = max|d| / max|d_prev| the
+            // observed contraction, the error left in X_{N_z} is about e = |d| rho / (1 - rho),
+            // so T = s^2 sum gL gR |X|^2 is off by at most dT = s^2 sum gL gR (2 |X| e + e^2)
+            // (open rows and columns).  Stop when dT <= tol.  The first iteration (no rho yet)
+            // stops only when the correction itself moves T by less than 1e-3 tol.
+            double dmx = 0.0, sxd = 0.0, sdd = 0.0;
             for (int e = t; e < nL * nR; e += blockDim.x) {
                 const int ai = e % nL, bi = e / nL;
                 const size_t o = ((size_t)(Nz - 1) * Np + a.Lrow[off + ai]) * ldb + bi;
                 const cfl d = Y[o];
                 const cd x = X[o];
-                dmx = fmax(dmx, fabs((double)d.x) + fabs((double)d.y));
-                xmx = fmax(xmx, fabs(x.x) + fabs(x.y));
+                const double ad = sqrt((double)d.x * d.x + (double)d.y * d.y), ax = sqrt(x.x * x.x + x.y * x.y);
+                const double gg = a.gL[off + ai] * a.gR[off + bi];
+                dmx = fmax(dmx, ad);
+                sxd += gg * ax * ad;
+                sdd += gg * ad * ad;
             }
             dmx = block_max2(dmx, s.red);
-            xmx = block_max2(xmx, s.red);
-            // the error left in X_{N_z} is about |d| rho / (1 - rho), rho = |d| / |d_prev| the
-            // observed contraction: stop when that is below tol |X|.  The first iteration
-            // (no rho yet) stops only on a correction below 1e-3 tol |X|.
+            sxd = block_sum2(sxd, s.red);
+            sdd = block_sum2(sdd, s.red);
+            const double s2 = sv.scale * sv.scale;
             const double rho = dprev > 0.0 ? dmx / dprev : 1.0;
-            conv = rho < 0.5 ? dmx * rho <= ma.tol * xmx * (1.0 - rho) : dmx <= 1e-3 * ma.tol * xmx;
+            const double f = rho / (1.0 - rho);
+            conv = rho < 0.5 ? s2 * (2.0 * f * sxd + f * f * sdd) <= ma.tol
+                             : s2 * (2.0 * sxd + sdd) <= 1e-3 * ma.tol;
             conv = __builtin_amdgcn_readfirstlane((int)conv) != 0;
             dprev = dmx;
+            // a contraction this slow will not reach the tolerance within the limit: the
+            // job goes to the FP64 path at once
+            if (!conv && it >= 3 && rho >= 0.9) break;
         }
         // 4. trace on X_{N_z} = G_{N_z,1}(:, open)
         double part = 0.0;
@@ -602,7 +640,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
         }
         __syncthreads();
     };
-    for (int job = fetch(); job < a.njobs; job = fetch()) run_job(job);
+    for (int c = fetch(); c < a.njobs; c = fetch()) run_job(__builtin_amdgcn_readfirstlane(ma.order[c]));
 }
 
 // D mode (diagonal blocks): backward generators stored per slot, then
@@ -1483,8 +1521,8 @@ struct negf_ctx {
     // reusable device buffers
     void *ws = nullptr;
     size_t ws_bytes = 0;
-    int mixed_refine = 12;  // refinement iteration limit of NEGF_PREC_MIXED_FP32
-    double mixed_tol = 1e-8; // its stopping tolerance (estimated relative error left in X_{N_z})
+    int mixed_refine = 24;  // refinement iteration limit of NEGF_PREC_MIXED_FP32
+    double mixed_tol = 1e-7; // its stopping tolerance (estimated absolute error of T)
     int last_fallbacks = 0; // energies of the last mixed call redone in FP64
     bool diag_one_sided = false; // transmission, D mode, bias window: never the two-sided forms (NEGF_DIAG_ONE_SIDED=1)
     bool mixed_fallback = true;  // mixed path: recompute unconverged energies in FP64 (NEGF_MIXED_FALLBACK=0 keeps
@@ -1688,7 +1726,11 @@ int slots_for(negf_ctx *c, const void *kernel, int threads, int njobs, size_t pe
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && per_slot_bytes > 0) {
         free_b += c->ws_bytes;   // the context's own workspace is reused (or released first)
-        const long fit = (long)((free_b * 0.8) / per_slot_bytes);
+        // keep 4 % (at least 4 GiB) free: at 256 slots of ~0.95 GB (the mixed path's
+        // generator history at N = 317, N_z = 1000) a 20 % margin cut the slots to ~240,
+        // and 1024 jobs then took five rounds on some workgroups instead of four
+        const size_t keep = std::max((size_t)4 << 30, free_b / 25);
+        const long fit = free_b > keep ? (long)((free_b - keep) / per_slot_bytes) : 1L;
         slots = std::min(slots, std::max(1L, fit));
     }
     return (int)std::max(1L, std::min<long>(slots, njobs));
@@ -1972,6 +2014,27 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
         ma.hsig = (int *)(base + b_hist * slots);
         ma.X = (cd *)(base + (b_hist + b_sig) * slots);
         ma.Y = (cfl *)(base + (b_hist + b_sig + b_x) * slots);
+        // Jobs near a subband threshold (an open channel with small q, or a closed one about
+        // to open) need the most refinement iterations; the kernel takes them first, so that
+        // none of them is started last.  Key: min over channels of |2 (E - V) - |G + k|^2|.
+        std::vector<int> order(nj);
+        {
+            std::vector<double> key(nj);
+            for (int j = 0; j < nj; ++j) {
+                const double Er = t.z[j].x;
+                double dmin = INFINITY;
+                for (int i = 0; i < c->N; ++i) {
+                    const double kn = 2.0 * t.kinh[(size_t)t.kidx[j] * c->Np + i];
+                    dmin = std::min(dmin, std::min(std::fabs(2.0 * (Er - c->VL) - kn), std::fabs(2.0 * (Er - c->VR) - kn)));
+                }
+                key[j] = dmin;
+                order[j] = j;
+            }
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
+        }
+        int *d_order;
+        if ((st = tmp.upload(&d_order, order.data(), nj, c->stream))) return st;
+        ma.order = d_order;
         if ((st = tmp.alloc(&ma.next, 1))) return st;
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, ma.next, 1);
         // each region holds `slots` padded per-slot blocks; the kernel steps through a region

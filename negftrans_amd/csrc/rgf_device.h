@@ -972,231 +972,284 @@ __device__ __forceinline__ int gje_invert4(typename GjePrec<R>::C *__restrict__ 
 }
 
 // ====================================================================================
-// Block Gauss-Jordan without row interchanges (the sweep of the mixed-precision path).
-// For each block K of 32 rows/columns [k0, k0 + 32):
-//     P = M_KK^{-1}                                  (FP64, by the wave owning the rows)
-//     M_IK <- -M_IK P  (I != K),   M_KK <- P          (matrix cores, precision R)
-//     M_IJ <- M_IJ + M_IK M_KJ,    M_KJ <- P M_KJ      (J != K; the trailing update)
-// This is the in-place Gauss-Jordan inversion of zgetri without pivoting: the 32 scalar
-// elimination steps of a panel become one 32 x 32 inverse and one 32 x 32 product per 32
-// rows.  The trailing update is gje_invert4's (old pivot rows staged in LDS straight from
-// Z, panel values as the B operand).  Stability rests on the diagonal dominance of the RGF
-// blocks D_m - g_{m+1}; the call returns nonzero when P is not finite, |P| exceeds
-// NEGF_BGJE_PMAX or a multiplier |M_IK P| exceeds NEGF_BGJE_MMAX, and the caller then
-// computes the job on the pivoted FP64 path instead.  No column map: inverse = Z.
+// Block Gauss-Jordan without row interchanges, FP32 on the matrix cores (the sweep of the
+// mixed-precision path).  Panels of 64 columns K = [k0, k0 + 64) (the last one 32 wide when
+// Np % 64 == 32) are factored as four 16-column sub-blocks K_s:
+//     P = M_{K_s K_s}^{-1}                          (FP64, 16 x 16, by the owner wave)
+//     M_{I K_s} <- -M_{I K_s} P (I != K_s),  M_{K_s K_s} <- P
+//     M_{I J} <- M_{I J} + M_{I K_s} M_{K_s J},  M_{K_s J} <- P M_{K_s J}   (J in K \ K_s)
+// all in registers (the panel) with the 16 pivot rows exchanged through LDS; the panel then
+// holds the Gauss-Jordan transform W of the whole 64-block, and the columns outside it get
+// the one rank-64 update  Z_{I J} <- [I not in K] Z_{I J} + W_I Z_{K J}  per panel: 5 read +
+// write passes over the block at Np = 320 instead of 10 with 32-column panels.  This is the
+// in-place inversion of zgetri without pivoting; its stability rests on the RGF blocks
+// D_m - g_{m+1}, and the call reports a failed growth check (P not finite, |P| above
+// NEGF_BGJE_PMAX, a multiplier |M_IK P| above NEGF_BGJE_MMAX) so that the caller can
+// compute the job on the pivoted FP64 path instead.  No column map: inverse = Z.
+//
+// Layout: wave w owns rows [32 w, 32 w + 32) as two 16-row tiles; lane l = j + 16 k holds
+// the panel values Pre/Pim[t][g][r] = Z[32 w + 16 t + j][k0 + 16 g + 4 r + k] (real and
+// imaginary parts).  That is the B operand of v_mfma_f32_16x16x4f32 for the K-step
+// q = 4 g + r (B[k][j] at lane j + 16 k), and also the D layout of a product whose A
+// operand rows are taken in the order c(i) = 4 (i % 4) + i / 4 (D[i][j] lands in lane
+// j + 16 (i / 4), register i % 4), so the sub-block products accumulate straight into
+// the panel registers.
 // ====================================================================================
 #define NEGF_BGJE_PMAX 1.0e6
 #define NEGF_BGJE_MMAX 1.0e3
 
-template <class R, int MAXNP = NEGF_MAXNP4, int TCH = NEGF_TCH>
+template <int MAXNP = NEGF_MAXNP4>
 struct BgjeShared {
-    typedef typename GjePrec<R>::C C;
-    static constexpr int PB = 32, MAXW = 2 * MAXNP / 64;
-    static constexpr int CH = TCH, LDT = TCH + 1;   // staged old pivot rows, padded stride
-    static constexpr int LDP = PB + 1;              // diagonal block, padded stride
-    C Tl[PB * LDT];
-    cd Pd[PB * LDP];     // M_KK, inverted in place in FP64: Pd[m * LDP + c]
+    static constexpr int CH = 64, LDT = CH + 1;     // staged old pivot rows (64 x CH), padded stride
+    static constexpr int LDP = 17, LDR = 65;        // 16 x 16 diagonal block, 16 pivot rows of the panel
+    static constexpr int MAXW = 2 * MAXNP / 64;
+    cfl Tl[64 * LDT];
+    cd Pd[2][16 * LDP];  // M_{K_s K_s}, inverted in place in FP64 (alternating by sub-block)
+    cfl Pr[2][16 * LDR]; // the sub-block's pivot rows over the panel columns, before the step
     int sigma[MAXNP];    // identity (the consumers read inverse[:, c] = Z[:, sigma[c]])
     int bad;
     int job;             // the workgroup's current job (kernels that take jobs in turn)
     double red[MAXW];
 };
 
-// In-place FP64 Gauss-Jordan inverse (no pivoting) of the 32 x 32 block A[r * lda + c] in
-// LDS by one wave: lane r + 32 h owns row r, columns [16 h, 16 h + 16).  Every lane reads
-// its step operands (pivot row, its row's column J) before any lane of the wave writes,
-// in program order, so the steps need no barrier.  bad: a pivot reciprocal beyond
+// In-place FP64 Gauss-Jordan inverse (no pivoting) of the 16 x 16 block A[r * 17 + c] in
+// LDS by one wave: lane r + 16 h owns row r, columns [4 h, 4 h + 4).  Every lane reads its
+// step operands before any lane of the wave writes (LDS operations of one wave are done in
+// program order), so the steps need no barrier.  bad: a pivot reciprocal beyond
 // NEGF_BGJE_PMAX (or not finite).
-__device__ __forceinline__ void bgje_diag_inverse(cd *A, const int lda, bool &bad) {
-    const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
-    cd *row = A + r * lda + 16 * h;
+__device__ __forceinline__ void bgje_diag16(cd *A, bool &bad) {
+    const int lane = threadIdx.x & 63, r = lane & 15, h = lane >> 4;
+    cd *row = A + r * 17 + 4 * h;
 #pragma unroll 1
-    for (int J = 0; J < 32; ++J) {
-        const cd *prow = A + J * lda + 16 * h;
-        const cd d = A[J * lda + J];
-        const cd m = A[r * lda + J];
+    for (int J = 0; J < 16; ++J) {
+        const cd *prow = A + J * 17 + 4 * h;
+        const cd d = A[J * 17 + J];
+        const cd m = A[r * 17 + J];
+        cd pv[4], x[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            pv[c] = prow[c];
+            x[c] = row[c];
+        }
         const cd inv = crecip_fast(d);
         if (!(cabs1(inv) <= NEGF_BGJE_PMAX)) bad = true;
         const bool piv = (r == J);
         const cd f = piv ? cmk(-inv.x, -inv.y) : cmul(m, inv);
-        // eight columns at a time: a chunk's stores touch only its own columns, so the
-        // next chunk's loads still see the step's old values
 #pragma unroll
-        for (int c0 = 0; c0 < 16; c0 += 8) {
-            cd pv[8], x[8];
+        for (int c = 0; c < 4; ++c) {
+            const cd n = cfms(f, pv[c], piv ? cmk(0.0, 0.0) : x[c]);
+            row[c] = (4 * h + c == J) ? cmk(-f.x, -f.y) : n;
+        }
+    }
+}
+
+// One sub-block step SB (static) of bgje_invert64's panel: see the comment above.
+template <int SB, class SH>
+__device__ __forceinline__ void bgje_substep(negf_f4 (&Pre)[2][4], negf_f4 (&Pim)[2][4], const int k0, const int nsb,
+                                             const int w, const int j, const int k, const int ci, SH &s, bool &bad) {
+    constexpr int TS = SB & 1, BUF = SB & 1;
+    typedef GjePrec<float> G;
+    const bool own = (w == (k0 >> 5) + (SB >> 1));      // wave-uniform
+    if (own) {
+        // publish the diagonal block in FP64 and the pivot rows over the panel, then invert
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                pv[c] = prow[c0 + c];
-                x[c] = row[c0 + c];
+        for (int r = 0; r < 4; ++r) s.Pd[BUF][j * SH::LDP + 4 * r + k] = cmk((double)Pre[TS][SB][r], (double)Pim[TS][SB][r]);
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) s.Pr[BUF][j * SH::LDR + 16 * g + 4 * r + k] = cmk(Pre[TS][g][r], Pim[TS][g][r]);
+        bgje_diag16(s.Pd[BUF], bad);
+    }
+    __syncthreads();
+    // W = -M_{I K_s} P for the tiles outside K_s, W = P for the owner tile
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt) {
+        if (own && tt == TS) {
+            float pm = 0.0f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const cd x = s.Pd[BUF][j * SH::LDP + 4 * r + k];
+                Pre[tt][SB][r] = (float)x.x;
+                Pim[tt][SB][r] = (float)x.y;
+                pm = fmaxf(pm, fabsf(Pre[tt][SB][r]) + fabsf(Pim[tt][SB][r]));
             }
+            if (!(pm <= (float)NEGF_BGJE_PMAX)) bad = true;
+        } else {
+            negf_f4 ar = {0.f, 0.f, 0.f, 0.f}, ai = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const cd n = cfms(f, pv[c], piv ? cmk(0.0, 0.0) : x[c]);
-                row[c0 + c] = (16 * h + c0 + c == J) ? cmk(-f.x, -f.y) : n;
+            for (int qq = 0; qq < 4; ++qq) {
+                const cd pd = s.Pd[BUF][(4 * qq + k) * SH::LDP + ci];
+                const float ax = (float)-pd.x, ay = (float)-pd.y;
+                const float bx = Pre[tt][SB][qq], by = Pim[tt][SB][qq];
+                ar = G::mfma(ax, bx, ar);
+                ar = G::mfma(-ay, by, ar);
+                ai = G::mfma(ax, by, ai);
+                ai = G::mfma(ay, bx, ai);
+            }
+            float vm = 0.0f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) vm = fmaxf(vm, fabsf(ar[r]) + fabsf(ai[r]));
+            if (!(vm <= (float)NEGF_BGJE_MMAX)) bad = true;
+            Pre[tt][SB] = ar;
+            Pim[tt][SB] = ai;
+        }
+    }
+    // the other sub-blocks of the panel: M_{I J} += W_I M_{K_s J}; the owner tile starts from 0
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        if (g == SB || g >= nsb) continue;
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt) {
+            const bool zero = own && tt == TS;
+            negf_f4 ar = zero ? negf_f4{0.f, 0.f, 0.f, 0.f} : Pre[tt][g];
+            negf_f4 ai = zero ? negf_f4{0.f, 0.f, 0.f, 0.f} : Pim[tt][g];
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) {
+                const cfl av = s.Pr[BUF][(4 * qq + k) * SH::LDR + 16 * g + ci];
+                const float bx = Pre[tt][SB][qq], by = Pim[tt][SB][qq];
+                ar = G::mfma(av.x, bx, ar);
+                ar = G::mfma(-av.y, by, ar);
+                ai = G::mfma(av.x, by, ai);
+                ai = G::mfma(av.y, bx, ai);
+            }
+            Pre[tt][g] = ar;
+            Pim[tt][g] = ai;
+        }
+    }
+}
+
+// One panel of bgje_invert64: NSB = 4 sub-blocks (64 columns from k0), or 2 for the last
+// panel when Np % 64 == 32.  The panel width is a template argument so that every panel
+// register is indexed statically.
+template <int NSB, class SH>
+__device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, const int k0, SH &s, bool &bad) {
+    typedef GjePrec<float> G;
+    typedef negf_f4 Acc;
+    constexpr int KW = 16 * NSB;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int j = lane & 15, k = lane >> 4;
+    const int R0 = w << 5;
+    const int ci = ((lane & 3) << 2) | ((lane >> 2) & 3);      // A row order c(i), i = lane & 15
+    const __amdgpu_buffer_rsrc_t zr = negf_rsrc(Z, (unsigned)(Np * Np) * 8u);
+    const unsigned cstr = (unsigned)Np * 8u;                   // one column
+    auto voff = [&](int row, int col) { return (unsigned)(row + col * Np) * 8u; };
+    negf_f4 Pre[2][4], Pim[2][4];
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            Pre[tt][g] = negf_f4{0.f, 0.f, 0.f, 0.f};
+            Pim[tt][g] = negf_f4{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+        for (int g = 0; g < NSB; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const cfl v = G::load(zr, voff(R0 + 16 * tt + j, k0 + 16 * g + 4 * r + k), 0u);
+                Pre[tt][g][r] = v.x;
+                Pim[tt][g][r] = v.y;
+            }
+    bgje_substep<0>(Pre, Pim, k0, NSB, w, j, k, ci, s, bad);
+    bgje_substep<1>(Pre, Pim, k0, NSB, w, j, k, ci, s, bad);
+    if constexpr (NSB == 4) {
+        bgje_substep<2>(Pre, Pim, k0, NSB, w, j, k, ci, s, bad);
+        bgje_substep<3>(Pre, Pim, k0, NSB, w, j, k, ci, s, bad);
+    }
+    // the panel columns take their final values
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+        for (int g = 0; g < NSB; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                G::store(cmk(Pre[tt][g][r], Pim[tt][g][r]), zr, voff(R0 + 16 * tt + j, k0 + 16 * g + 4 * r + k), 0u);
+    // trailing update, one chunk of CH columns at a time: the old pivot rows Z_{K J} are
+    // staged from Z (their new values are written only by the chunk's own update, after
+    // the staging barrier); rows in K start from zero
+    const bool zb = (R0 >= k0 && R0 < k0 + KW);
+    auto next_tile = [&](int ct) {
+        const int n = (ct + 16 == k0) ? k0 + 64 : ct + 16;
+        return n < Np ? n : ct;
+    };
+    Acc nb0r, nb0i, nb1r, nb1i;
+    auto load_base = [&](int ct) {
+        const unsigned vo = voff(R0 + j, ct + 4 * k);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const cfl a0 = G::load(zr, vo, r * cstr), a1 = G::load(zr, vo + 128u, r * cstr);
+            nb0r[r] = a0.x;
+            nb0i[r] = a0.y;
+            nb1r[r] = a1.x;
+            nb1i[r] = a1.y;
+        }
+    };
+    load_base(k0 == 0 ? 64 : 0);
+    for (int c0 = 0; c0 < Np; c0 += SH::CH) {
+        if (c0 == k0) continue;                              // the panel's own columns
+        const int c1 = min(c0 + SH::CH, Np);
+        for (int e = t; e < (c1 - c0) * KW; e += blockDim.x) {
+            const int pp = e % KW, cl = e / KW;
+            s.Tl[pp * SH::LDT + cl] = Z[(k0 + pp) + (size_t)(c0 + cl) * Np];
+        }
+        __syncthreads();
+        for (int ct = c0; ct < c1; ct += 16) {
+            // 3M complex products: with A = T (staged pivot rows) and B = W (panel),
+            //   c1 += Ar Br,  c2 += Ai Bi,  c3 += (Ar + Ai)(Br + Bi),
+            //   re = c1 - c2,  im = c3 - c1 - c2,
+            // from c1 = base re, c2 = 0, c3 = base re + base im: three matrix-core products per
+            // complex K-step instead of four
+            Acc c10, c20, c30, c11, c21, c31;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                c10[r] = zb ? 0.f : nb0r[r];
+                c30[r] = zb ? 0.f : nb0r[r] + nb0i[r];
+                c11[r] = zb ? 0.f : nb1r[r];
+                c31[r] = zb ? 0.f : nb1r[r] + nb1i[r];
+                c20[r] = 0.f;
+                c21[r] = 0.f;
+            }
+            load_base(next_tile(ct));
+            __builtin_amdgcn_sched_barrier(0);          // keep the prefetch ahead of the MFMAs
+            int lx = threadIdx.x & 63;
+            asm volatile("" : "+v"(lx));                 // recompute, do not keep live
+            const cfl *tl = s.Tl + (lx >> 4) * SH::LDT + (ct - c0 + (lx & 15));
+#pragma unroll
+            for (int q = 0; q < 4 * NSB; ++q) {
+                const cfl tv = tl[4 * q * SH::LDT];
+                const float ts = tv.x + tv.y;
+                const float p0r = Pre[0][q >> 2][q & 3], p0i = Pim[0][q >> 2][q & 3];
+                const float p1r = Pre[1][q >> 2][q & 3], p1i = Pim[1][q >> 2][q & 3];
+                c10 = G::mfma(tv.x, p0r, c10);
+                c11 = G::mfma(tv.x, p1r, c11);
+                c20 = G::mfma(tv.y, p0i, c20);
+                c21 = G::mfma(tv.y, p1i, c21);
+                c30 = G::mfma(ts, p0r + p0i, c30);
+                c31 = G::mfma(ts, p1r + p1i, c31);
+            }
+            const unsigned vo = voff(R0 + j, ct + 4 * k);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                G::store(cmk(c10[r] - c20[r], c30[r] - c10[r] - c20[r]), zr, vo, r * cstr);
+                G::store(cmk(c11[r] - c21[r], c31[r] - c11[r] - c21[r]), zr, vo + 128u, r * cstr);
             }
         }
+        __syncthreads();
     }
 }
 
 // In-place inversion of Z (Np x Np, Np % 32 == 0, blockDim.x == 2 Np); on return
 // inverse = Z and s.sigma is the identity.  Returns 0, or 1 if the growth check failed.
-template <class R, class SH>
-__device__ __forceinline__ int bgje_invert4(typename GjePrec<R>::C *__restrict__ Z, const int Np, SH &s) {
-    typedef GjePrec<R> G;
-    typedef typename G::C C;
-    typedef typename G::Acc Acc;
-    constexpr int PB = 32, KK = 8, LDP = SH::LDP;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int j = lane & 15, k = lane >> 4;
-    const int R0 = w << 5;
-    const size_t ld = (size_t)Np;
+template <class SH>
+__device__ __forceinline__ int bgje_invert64(cfl *__restrict__ Z, const int Np, SH &s) {
+    const int t = threadIdx.x, lane = t & 63;
     for (int i = t; i < Np; i += blockDim.x) s.sigma[i] = i;
     if (t == 0) s.bad = 0;
     bool bad = false;
-    for (int k0 = 0; k0 < Np; k0 += PB) {
-        C P[2][KK];
-#pragma unroll
-        for (int q = 0; q < KK; ++q) {
-            const C *zc = Z + (R0 + j) + (size_t)(k0 + 4 * q + k) * ld;
-            P[0][q] = zc[0];
-            P[1][q] = zc[16];
-        }
-        const bool owner = (R0 == k0);
-        if (owner) {
-            // P = M_KK^{-1} in FP64; the owner's rows are the pivot rows: they take P
-#pragma unroll
-            for (int tt = 0; tt < 2; ++tt)
-#pragma unroll
-                for (int q = 0; q < KK; ++q) s.Pd[(16 * tt + j) * LDP + 4 * q + k] = to_cd(P[tt][q]);
-            bgje_diag_inverse(s.Pd, LDP, bad);
-            double pm = 0.0;
-#pragma unroll
-            for (int tt = 0; tt < 2; ++tt)
-#pragma unroll
-                for (int q = 0; q < KK; ++q) {
-                    const cd x = s.Pd[(16 * tt + j) * LDP + 4 * q + k];
-                    pm = fmax(pm, cabs1(x));
-                    P[tt][q] = from_cd<C>(x);
-                }
-            if (!(pm <= NEGF_BGJE_PMAX)) bad = true;
-        }
-        __syncthreads();
-        if (!owner) {
-            // panel values -M_IK P: D = A B on the matrix cores with A[i][kk] = -P[4 q + kk][c(i)],
-            // c(i) = 16 h + 4 (i % 4) + i / 4, and B[kk][jj] = M_IK[16 t + jj][4 q + kk] = P[t][q];
-            // D[i][jj] lands in lane jj + 16 (i / 4), register i % 4, i.e. panel column
-            // 16 h + 4 r + k of lane j + 16 k, register r: the panel layout again (q = 4 h + r).
-            // (FP64 accumulators hold row i in lane group i % 4, register i / 4: c(i) = 16 h + i)
-            const int ci = G::CQ == 1 ? (lane & 15) : (((lane & 3) << 2) | ((lane >> 2) & 3));
-            C Pv[2][KK];
-#pragma unroll
-            for (int tt = 0; tt < 2; ++tt)
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    Acc ar, ai;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        ar[r] = R(0);
-                        ai[r] = R(0);
-                    }
-#pragma unroll
-                    for (int q = 0; q < KK; ++q) {
-                        const cd pd = s.Pd[(4 * q + k) * LDP + 16 * h + ci];
-                        const C a = from_cd<C>(cmk(-pd.x, -pd.y));
-                        const C b = P[tt][q];
-                        ar = G::mfma(a.x, b.x, ar);
-                        ar = G::mfma(-a.y, b.y, ar);
-                        ai = G::mfma(a.x, b.y, ai);
-                        ai = G::mfma(a.y, b.x, ai);
-                    }
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) Pv[tt][4 * h + r] = cmk(ar[r], ai[r]);
-                }
-            R vmax = R(0);
-#pragma unroll
-            for (int tt = 0; tt < 2; ++tt)
-#pragma unroll
-                for (int q = 0; q < KK; ++q) {
-                    P[tt][q] = Pv[tt][q];
-                    const R v = cabs1(Pv[tt][q]);
-                    vmax = v > vmax ? v : vmax;
-                }
-            if (!(vmax <= R(NEGF_BGJE_MMAX))) bad = true;
-        }
-        // panel columns take their final values
-#pragma unroll
-        for (int q = 0; q < KK; ++q) {
-            C *zc = Z + (R0 + j) + (size_t)(k0 + 4 * q + k) * ld;
-            zc[0] = P[0][q];
-            zc[16] = P[1][q];
-        }
-        // trailing update on the matrix cores, one chunk of SH::CH columns at a time: the
-        // old pivot rows M_KJ are staged from Z (their new values are written only by the
-        // chunk's own update, after the staging barrier); the pivot rows start from zero
-        const bool zb = owner;
-        const __amdgpu_buffer_rsrc_t zr = negf_rsrc(Z, (unsigned)(Np * Np) * G::ESZ);
-        const unsigned cstr = (unsigned)G::CR * (unsigned)Np * G::ESZ;
-        auto next_tile = [&](int ct) {
-            const int n = (ct + 16 == k0) ? k0 + PB : ct + 16;
-            return n < Np ? n : ct;
-        };
-        auto voff = [&](int ct) { return (unsigned)((R0 + j) + (ct + G::CQ * k) * Np) * G::ESZ; };
-        C nb0[4], nb1[4];
-        auto load_base = [&](int ct) {
-            const unsigned vo = voff(ct);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                nb0[q] = G::load(zr, vo, q * cstr);
-                nb1[q] = G::load(zr, vo + 16u * G::ESZ, q * cstr);
-            }
-        };
-        load_base(k0 == 0 ? PB : 0);
-        for (int c0 = 0; c0 < Np; c0 += SH::CH) {
-            const int c1 = min(c0 + SH::CH, Np);
-            if (c0 >= k0 && c1 <= k0 + PB) continue;       // chunk is all panel
-            for (int e = t; e < (c1 - c0) * PB; e += blockDim.x) {
-                const int pp = e % PB, cl = e / PB;
-                const int col = c0 + cl;
-                if (!(col >= k0 && col < k0 + PB)) s.Tl[pp * SH::LDT + cl] = Z[(k0 + pp) + (size_t)col * ld];
-            }
-            __syncthreads();
-            for (int ct = c0; ct < c1; ct += 16) {
-                if (ct >= k0 && ct < k0 + PB) continue;
-                Acc ar0, ai0, ar1, ai1;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    ar0[q] = zb ? R(0) : nb0[q].x;
-                    ai0[q] = zb ? R(0) : nb0[q].y;
-                    ar1[q] = zb ? R(0) : nb1[q].x;
-                    ai1[q] = zb ? R(0) : nb1[q].y;
-                }
-                load_base(next_tile(ct));
-                __builtin_amdgcn_sched_barrier(0);          // keep the prefetch ahead of the MFMAs
-                int lx = threadIdx.x & 63;
-                asm volatile("" : "+v"(lx));                 // recompute, do not keep live
-                const C *tl = s.Tl + (lx >> 4) * SH::LDT + (ct - c0 + (lx & 15));
-#pragma unroll
-                for (int q = 0; q < KK; ++q) {
-                    const C tv = tl[4 * q * SH::LDT];
-                    const R ntim = -tv.y;
-                    ar0 = G::mfma(tv.x, P[0][q].x, ar0);
-                    ar1 = G::mfma(tv.x, P[1][q].x, ar1);
-                    ai0 = G::mfma(tv.y, P[0][q].x, ai0);
-                    ai1 = G::mfma(tv.y, P[1][q].x, ai1);
-                    ar0 = G::mfma(ntim, P[0][q].y, ar0);
-                    ar1 = G::mfma(ntim, P[1][q].y, ar1);
-                    ai0 = G::mfma(tv.x, P[0][q].y, ai0);
-                    ai1 = G::mfma(tv.x, P[1][q].y, ai1);
-                }
-                const unsigned vo = voff(ct);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    G::store(cmk(ar0[q], ai0[q]), zr, vo, q * cstr);
-                    G::store(cmk(ar1[q], ai1[q]), zr, vo + 16u * G::ESZ, q * cstr);
-                }
-            }
-            __syncthreads();
-        }
+    for (int k0 = 0; k0 < Np; k0 += 64) {
+        if (k0 + 64 <= Np) bgje_panel<4>(Z, Np, k0, s, bad);
+        else bgje_panel<2>(Z, Np, k0, s, bad);
     }
     if (__ballot(bad) && lane == 0) atomicOr(&s.bad, 1);
     __syncthreads();

@@ -3,7 +3,8 @@
 k_lead_decimation replaces the reference's closed-form exp(i q Delta) by the surface term
 of a semi-infinite flat lead on the device's own lattice.  Bars, each written in its
 assertion: relative 1e-10 against the oracle (same decimation restated in numpy) for
-transmission, density and LDOS; 1e-9 for the mixed path; a V = 0 wire transmits the
+transmission, density and LDOS; 1e-9 for the mixed path at refinement tolerance 1e-13;
+a V = 0 wire transmits the
 number of open lattice channels to 1e-5 (eta = 1e-7).
 """
 import dataclasses
@@ -79,12 +80,14 @@ def test_decimation_density_and_ldos(ng):
         assert rel_err(L[:, :, :, i], np.moveaxis(O.ldos_matrix(s, e), 0, 2)) < RTOL
 
 
-def test_decimation_mixed_path(ng):
+def test_decimation_mixed_path(ng, monkeypatch):
     s, _ = small_system(N_z=40, seed=5)
     s = dataclasses.replace(s, lead="decimation")
     E = np.linspace(0.05, 0.6, 13)
     ref = O.transmission(s, E)
+    monkeypatch.setenv("NEGF_MIXED_TOL", "1e-13")
     dev = dec_device(ng, s, prec=ng.PREC_MIXED_FP32)
+    monkeypatch.delenv("NEGF_MIXED_TOL")
     got = dev.transmission(E)
     assert dev.last_status == 0
     assert np.all(np.abs(got - ref) <= 1e-9 * (1.0 + np.abs(ref)))
