@@ -469,12 +469,24 @@ template <int MAXT>
 __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
     __shared__ BgjeShared<> s;
     __shared__ SolveShared so;
+    __shared__ int ckey[NEGF_MAXNP4];
     const TransArgs &a = ma.a;
     const SysView &sv = a.sv;
     const int Np = sv.Np, Nz = sv.Nz, ldb = ma.ldb;
     const size_t NN = (size_t)Np * Np;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int jj = lane & 15, q = lane >> 4;
+    // basis keys for the V-hat window index of the block build, and the slice's window
+    // staged in LDS when it fits (the Gauss-Jordan's staging area, free outside the
+    // inversion); otherwise read from HBM
+    const int nvh = (2 * sv.NGX + 1) * (2 * sv.NGY + 1);
+    const bool vh_lds = nvh <= BgjeShared<>::VHCAP;
+    for (int c = t; c < Np; c += blockDim.x) ckey[c] = c < sv.N ? sv.nx[c] + (2 * sv.NGX + 1) * sv.ny[c] : 0;
+    auto stage_vh = [&](int slice) {             // the caller synchronizes
+        if (vh_lds)
+            for (int e = t; e < nvh; e += blockDim.x) s.vh[e] = sv.vrec[(size_t)slice * nvh + e];
+    };
+    auto vsrc = [&](int slice) -> const cd * { return vh_lds ? s.vh : sv.vrec + (size_t)slice * nvh; };
     cfl *hist = ma.hist + (size_t)blockIdx.x * Nz * NN;
     int *hsig = ma.hsig + (size_t)blockIdx.x * Nz * Np;
     cd *X = ma.X + (size_t)blockIdx.x * Nz * Np * ldb;
@@ -514,7 +526,9 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
         __syncthreads();
         for (int m = Nz; m >= 1 && !grew; --m) {
             cfl *M = hist + (size_t)(m - 1) * NN;
-            build_block<cfl>(M, sv, jv, m - 1, m == Nz ? nullptr : hist + (size_t)m * NN, nullptr);
+            stage_vh(m - 1);
+            __syncthreads();
+            build_block_keys(M, sv, jv, m - 1, m == Nz ? nullptr : hist + (size_t)m * NN, vsrc(m - 1), ckey);
             __syncthreads();
             if (bgje_invert64(M, Np, s)) grew = true;
             for (int i = t; i < Np; i += blockDim.x) hsig[(size_t)(m - 1) * Np + i] = i;

@@ -224,6 +224,47 @@ __device__ __forceinline__ void build_block(C *__restrict__ M, const SysView &sv
     }
 }
 
+// build_block for the mixed sweep (M = D_slice - S, FP64 arithmetic rounded to complex64),
+// with the slice's V-hat window read through vs (LDS or global) and the basis keys
+// ckey[c] = nx_c + (2 NGX + 1) ny_c (c < N) from LDS: the window index of (r, c) is
+// ckey[r] - ckey[c] + NGX + (2 NGX + 1) NGY.
+__device__ __forceinline__ void build_block_keys(cfl *__restrict__ M, const SysView &sv, const JobView &jv,
+                                                 const int slice, const cfl *__restrict__ S, const cd *vs,
+                                                 const int *ckey) {
+    const int Np = sv.Np, N = sv.N;
+    const int base0 = sv.NGX + (2 * sv.NGX + 1) * sv.NGY;
+    constexpr int CBB = 8;
+    const int nitems = Np * (Np / CBB);
+#pragma unroll 2
+    for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
+        const int r = it % Np, c0 = (it / Np) * CBB;
+        const bool rin = r < N;
+        const int rb = rin ? ckey[r] + base0 : 0;
+        const double kin = rin ? jv.kinh[r] : 0.0;
+        cd dg = cmk(0.0, 0.0);
+        if (rin) {
+            if (slice == 0) dg = cadd(dg, jv.sigL[r]);
+            if (slice == sv.Nz - 1) dg = cadd(dg, jv.sigR[r]);
+        }
+        cfl sp[CBB];
+        cd hv[CBB];
+#pragma unroll
+        for (int q = 0; q < CBB; ++q) {
+            const int col = c0 + q;
+            sp[q] = S ? S[r + (size_t)col * Np] : cmk(0.0f, 0.0f);
+            hv[q] = (rin && col < N) ? vs[rb - ckey[col]] : cmk(0.0, 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < CBB; ++q) {
+            const int col = c0 + q;
+            cd v;
+            if (rin && col < N) v = csub(d_entry(hv[q], r == col, kin, jv.z, sv.scale, dg), to_cd(sp[q]));
+            else v = cmk(r == col ? 1.0 : 0.0, 0.0);
+            M[r + (size_t)col * Np] = from_cd<cfl>(v);
+        }
+    }
+}
+
 // dst = S(:, sig) (column gather), Np x Np.
 __device__ __forceinline__ void copy_cols(cd *__restrict__ dst, const cd *__restrict__ S, const int *sig, int Np) {
     for (int col = 0; col < Np; ++col) {
@@ -1003,9 +1044,15 @@ struct BgjeShared {
     static constexpr int CH = 64, LDT = CH + 1;     // staged old pivot rows (64 x CH), padded stride
     static constexpr int LDP = 17, LDR = 65;        // 16 x 16 diagonal block, 16 pivot rows of the panel
     static constexpr int MAXW = 2 * MAXNP / 64;
-    cfl Tl[64 * LDT];
-    cd Pd[2][16 * LDP];  // M_{K_s K_s}, inverted in place in FP64 (alternating by sub-block)
-    cfl Pr[2][16 * LDR]; // the sub-block's pivot rows over the panel columns, before the step
+    static constexpr int VHCAP = (64 * LDT * 8 + 2 * 16 * LDP * 16 + 2 * 16 * LDR * 8) / 16;
+    union {
+        struct {
+            cfl Tl[64 * LDT];
+            cd Pd[2][16 * LDP];  // M_{K_s K_s}, inverted in place in FP64 (alternating by sub-block)
+            cfl Pr[2][16 * LDR]; // the sub-block's pivot rows over the panel columns, before the step
+        };
+        cd vh[VHCAP];            // outside the inversion: one slice's V-hat window (the kernel's)
+    };
     int sigma[MAXNP];    // identity (the consumers read inverse[:, c] = Z[:, sigma[c]])
     int bad;
     int job;             // the workgroup's current job (kernels that take jobs in turn)
