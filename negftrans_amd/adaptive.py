@@ -146,10 +146,23 @@ def dataset(seeds, n_bins, log_every=0, **kw):
     return np.array(X, dtype=np.float32), np.array(T, dtype=np.float32)
 
 
-def judge(model, seeds, n_bins, n_dense, tol=1e-4, n0=48, fallback_tol=0.25, log_every=0, **kw):
+def uniform_points_needed(dev, wl, G_ref, tol, E_lo, E_hi, n_start=65, n_max=8193):
+    """The smallest uniform grid of n = 64 * 2^j + 1 points on [E_lo, E_hi] (j = 0, 1, ...,
+    up to n_max) whose trapezoid G is within tol of G_ref; n_max + 1 if none is."""
+    n = n_start
+    while n <= n_max:
+        Eu = np.linspace(E_lo, E_hi, n)
+        if abs(integrated(Eu, transmission_at(dev, wl, Eu)) - G_ref) <= tol:
+            return n
+        n = 2 * n - 1
+    return n_max + 1
+
+
+def judge(model, seeds, n_bins, n_dense, tol=1e-4, n0=48, fallback_tol=0.25, log_every=0, uniform_search=False, **kw):
     """Per held-out device: the dense G; the network-seeded adaptive G and its RGF count;
     the same refinement started from a uniform mesh of n0 points (the method without the
-    network) and its count; a uniform grid with the network-seeded count."""
+    network) and its count; a uniform grid with the network-seeded count; with
+    uniform_search, also the smallest uniform grid (64 * 2^j + 1 points) that meets tol."""
     rows = []
     for i, s in enumerate(seeds):
         if log_every and i % log_every == 0:
@@ -168,6 +181,8 @@ def judge(model, seeds, n_bins, n_dense, tol=1e-4, n0=48, fallback_tol=0.25, log
         rows.append(dict(seed=int(s), G_dense=G_dense, G_adaptive=G_ad, points=n_pts, fallback=bool(fb),
                          G_refine_uniform_start=integrated(Eb, Tb), points_uniform_start=len(Eb),
                          G_uniform=integrated(Eu, transmission_at(dev, wl, Eu))))
+        if uniform_search:
+            rows[-1]["uniform_points_needed"] = uniform_points_needed(dev, wl, G_dense, tol, E_lo, E_hi)
         dev.close()
     return rows
 
@@ -178,7 +193,11 @@ def summary(rows, n_dense):
     ea, eb, eu = err("G_adaptive"), err("G_refine_uniform_start"), err("G_uniform")
     pa = np.array([r["points"] for r in rows])
     pb = np.array([r["points_uniform_start"] for r in rows])
-    return dict(devices=len(rows), dense_points=n_dense, fallback_rate=float(np.mean([r["fallback"] for r in rows])),
+    out = {}
+    if rows and "uniform_points_needed" in rows[0]:
+        pu = np.array([r["uniform_points_needed"] for r in rows])
+        out = dict(uniform_needed_mean=float(pu.mean()), uniform_needed_max=int(pu.max()))
+    return dict(out, devices=len(rows), dense_points=n_dense, fallback_rate=float(np.mean([r["fallback"] for r in rows])),
                 points_mean=float(pa.mean()), points_max=int(pa.max()),
                 err_median=float(np.median(ea)), err_max=float(ea.max()), within_1e_4=float(np.mean(ea <= 1e-4)),
                 no_network_points_mean=float(pb.mean()), no_network_err_median=float(np.median(eb)),
@@ -197,9 +216,13 @@ def main(argv=None):
     ap.add_argument("--n0", type=int, default=48)
     ap.add_argument("--N_z", type=int, default=400)
     ap.add_argument("--R2", type=float, default=36.5)
+    ap.add_argument("--sigma", type=float, nargs=2, default=(0.0, 0.0005),
+                    help="range (eV) of the disorder strength drawn per device")
+    ap.add_argument("--uniform-search", action="store_true",
+                    help="also find the smallest uniform grid that meets the tolerance")
     ap.add_argument("--out", default="")
     a = ap.parse_args(argv)
-    kw = dict(N_z=a.N_z, R2=a.R2)
+    kw = dict(N_z=a.N_z, R2=a.R2, sigma=tuple(a.sigma))
     t0 = time.time()
     X, T = dataset(range(1000, 1000 + a.train), a.bins, log_every=64, **kw)
     t1 = time.time()
@@ -208,7 +231,8 @@ def main(argv=None):
     hist = model.fit(X, T, epochs=a.epochs, batch=64, lr=1e-3, log_every=max(1, a.epochs // 10))
     t2 = time.time()
     print(f"# training: {a.epochs} epochs in {t2 - t1:.1f} s, loss {hist[0]:.4f} -> {hist[-1]:.4f}", flush=True)
-    rows = judge(model, range(900000, 900000 + a.test), a.bins, a.dense, a.tol, a.n0, log_every=4, **kw)
+    rows = judge(model, range(900000, 900000 + a.test), a.bins, a.dense, a.tol, a.n0, log_every=4,
+                 uniform_search=a.uniform_search, **kw)
     t3 = time.time()
     res = dict(summary(rows, a.dense), train_devices=a.train, epochs=a.epochs, loss_first=hist[0],
                loss_last=hist[-1], seconds=dict(data=t1 - t0, train=t2 - t1, judge=t3 - t2), rows=rows)

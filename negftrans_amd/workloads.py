@@ -24,8 +24,10 @@ Configurations (all single-device, k = 0):
   qpc_c4_instance(seed)  the learned adaptive grid's instances: qpc_c0's form with
             N = 113 (R^2 = 36.5), N_z = 400 slices of 0.25 nm on a 32 x 32 grid, seeded
             V0 in 6-14 meV, w in 6-10 nm, saddle centre 40-60 nm and width 8-12 nm,
-            confinement 1.5-2.5 meV, plus Gaussian-correlated disorder (sigma 0-0.5 meV,
-            5 nm); energies in 0-20 meV.  Features: the 16 x 16 x 8 lowest Fourier
+            confinement 1.5-2.5 meV, plus Gaussian-correlated disorder (sigma drawn from
+            0-0.5 meV by default, or the range `sigma` in eV; 5 nm); energies in 0-20 meV.
+            From about 2 meV of disorder on, T(E) has resonances that uniform grids
+            resolve only with thousands of points.  Features: the 16 x 16 x 8 lowest Fourier
             components of V, returned as the band-limited real samples they define.
 """
 from __future__ import annotations
@@ -127,7 +129,7 @@ def disorder_c1(n_energy=4096, N_z=1000, R2=100.5, grid=64, sigma=0.003, seed=20
     return Workload("disorder_c1", inp, ms * V, E)
 
 
-def qpc_c4_instance(seed, n_energy=512, N_z=400, R2=36.5, grid=32) -> Workload:
+def qpc_c4_instance(seed, n_energy=512, N_z=400, R2=36.5, grid=32, sigma=(0.0, 0.0005)) -> Workload:
     rng = np.random.default_rng(seed)
     Lx = Ly = 400.0
     LZ = 1000.0
@@ -141,7 +143,7 @@ def qpc_c4_instance(seed, n_energy=512, N_z=400, R2=36.5, grid=32) -> Workload:
     hw = rng.uniform(0.0015, 0.0025)
     saddle = V0 * np.exp(-(z - z0) ** 2 / (2 * zw ** 2)) * (x - Lx / 2) ** 2 / w ** 2
     conf = 0.5 * hw ** 2 / (HBAR2_OVER_ME / ms) * (y - Ly / 2) ** 2
-    dis = gaussian_disorder(grid, N_z, Lx, dz, rng.uniform(0.0, 0.0005), 50.0, seed + 7919, 100.0)
+    dis = gaussian_disorder(grid, N_z, Lx, dz, rng.uniform(*sigma), 50.0, seed + 7919, 100.0)
     V = saddle + conf + dis
     inp = dict(if_transmission=True, LX=Lx, LY=Ly, LZ=LZ, NKX=1, NKY=1, ENCUT=encut_for(R2, Lx),
                ETA=ms * 1e-6 / HARTREE, TRANSMISSION_GRID=(0.0, ms * 0.020, n_energy))
