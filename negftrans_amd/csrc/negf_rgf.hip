@@ -315,17 +315,21 @@ struct SolveShared {
 // (A[i][k] at lane i + 16 k) and B = cL (B[k][b] at lane b + 16 k); lane b + 16 q,
 // register r of tile t holds out(32 w + 16 t + 4 q + r, b).
 // The generator is streamed from HBM in groups of 8 K-steps (32 columns) with the next
-// group's loads issued before the current group's 64 MFMAs (two register buffers), so the
-// load latency hides behind the matrix cores; Np is a multiple of 32.
+// group's loads issued before the current group's MFMAs (two register buffers), so the
+// load latency hides behind the matrix cores; Np is a multiple of 32.  Complex products in
+// the 3M form: p1 += Ar Br, p2 += Ai Bi, p3 += (Ar + Ai)(Br + Bi); re = p1 - p2,
+// im = p3 - p1 - p2 (three MFMAs per tile and K-step instead of four).
 __device__ __forceinline__ void gen_matvec(const cfl *Zg, const cfl *cL, int Np, negf_f4 (&ore)[2],
                                            negf_f4 (&oim)[2]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = lane & 15, k = lane >> 4;
     const __amdgpu_buffer_rsrc_t zr = negf_rsrc(const_cast<cfl *>(Zg), (unsigned)(Np * Np) * 8u);
     const unsigned vo = (unsigned)(((w << 5) + i) + k * Np) * 8u;
+    negf_f4 p1[2], p2[2], p3[2];
     for (int tt = 0; tt < 2; ++tt) {
-        ore[tt] = negf_f4{0.f, 0.f, 0.f, 0.f};
-        oim[tt] = negf_f4{0.f, 0.f, 0.f, 0.f};
+        p1[tt] = negf_f4{0.f, 0.f, 0.f, 0.f};
+        p2[tt] = negf_f4{0.f, 0.f, 0.f, 0.f};
+        p3[tt] = negf_f4{0.f, 0.f, 0.f, 0.f};
     }
     constexpr int GK = 8;                       // K-steps (of 4 columns) per group
     const unsigned gstr = 4u * (unsigned)Np * 8u;
@@ -341,15 +345,13 @@ __device__ __forceinline__ void gen_matvec(const cfl *Zg, const cfl *cL, int Np,
 #pragma unroll
         for (int s = 0; s < GK; ++s) {
             const cfl b = cL[((g * GK + s) * 4 + k) * NEGF_NRH + i];
-            const float nby = -b.y;
-            ore[0] = GjePrec<float>::mfma(A[s][0].x, b.x, ore[0]);
-            ore[1] = GjePrec<float>::mfma(A[s][1].x, b.x, ore[1]);
-            oim[0] = GjePrec<float>::mfma(A[s][0].x, b.y, oim[0]);
-            oim[1] = GjePrec<float>::mfma(A[s][1].x, b.y, oim[1]);
-            ore[0] = GjePrec<float>::mfma(A[s][0].y, nby, ore[0]);
-            ore[1] = GjePrec<float>::mfma(A[s][1].y, nby, ore[1]);
-            oim[0] = GjePrec<float>::mfma(A[s][0].y, b.x, oim[0]);
-            oim[1] = GjePrec<float>::mfma(A[s][1].y, b.x, oim[1]);
+            const float bs = b.x + b.y;
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt) {
+                p1[tt] = GjePrec<float>::mfma(A[s][tt].x, b.x, p1[tt]);
+                p2[tt] = GjePrec<float>::mfma(A[s][tt].y, b.y, p2[tt]);
+                p3[tt] = GjePrec<float>::mfma(A[s][tt].x + A[s][tt].y, bs, p3[tt]);
+            }
         }
     };
     const int ng = Np / (4 * GK);
@@ -364,6 +366,11 @@ __device__ __forceinline__ void gen_matvec(const cfl *Zg, const cfl *cL, int Np,
             __builtin_amdgcn_sched_barrier(0);
             compute(A1, g + 1);
         }
+    }
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt) {
+        ore[tt] = p1[tt] - p2[tt];
+        oim[tt] = p3[tt] - p1[tt] - p2[tt];
     }
 }
 

@@ -1122,17 +1122,18 @@ __device__ __forceinline__ void bgje_substep(negf_f4 (&Pre)[2][4], negf_f4 (&Pim
             }
             if (!(pm <= (float)NEGF_BGJE_PMAX)) bad = true;
         } else {
-            negf_f4 ar = {0.f, 0.f, 0.f, 0.f}, ai = {0.f, 0.f, 0.f, 0.f};
+            // 3M: c1 += Ar Br, c2 += Ai Bi, c3 += (Ar + Ai)(Br + Bi); re = c1 - c2, im = c3 - c1 - c2
+            negf_f4 c1 = {0.f, 0.f, 0.f, 0.f}, c2 = {0.f, 0.f, 0.f, 0.f}, c3 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int qq = 0; qq < 4; ++qq) {
                 const cd pd = s.Pd[BUF][(4 * qq + k) * SH::LDP + ci];
                 const float ax = (float)-pd.x, ay = (float)-pd.y;
                 const float bx = Pre[tt][SB][qq], by = Pim[tt][SB][qq];
-                ar = G::mfma(ax, bx, ar);
-                ar = G::mfma(-ay, by, ar);
-                ai = G::mfma(ax, by, ai);
-                ai = G::mfma(ay, bx, ai);
+                c1 = G::mfma(ax, bx, c1);
+                c2 = G::mfma(ay, by, c2);
+                c3 = G::mfma(ax + ay, bx + by, c3);
             }
+            const negf_f4 ar = c1 - c2, ai = c3 - c1 - c2;
             float vm = 0.0f;
 #pragma unroll
             for (int r = 0; r < 4; ++r) vm = fmaxf(vm, fabsf(ar[r]) + fabsf(ai[r]));
@@ -1148,19 +1149,21 @@ __device__ __forceinline__ void bgje_substep(negf_f4 (&Pre)[2][4], negf_f4 (&Pim
 #pragma unroll
         for (int tt = 0; tt < 2; ++tt) {
             const bool zero = own && tt == TS;
-            negf_f4 ar = zero ? negf_f4{0.f, 0.f, 0.f, 0.f} : Pre[tt][g];
-            negf_f4 ai = zero ? negf_f4{0.f, 0.f, 0.f, 0.f} : Pim[tt][g];
+            const negf_f4 z4 = {0.f, 0.f, 0.f, 0.f};
+            // 3M, from c1 = base re, c2 = 0, c3 = base re + base im (zero for the owner tile)
+            negf_f4 c1 = zero ? z4 : Pre[tt][g];
+            negf_f4 c3 = zero ? z4 : Pre[tt][g] + Pim[tt][g];
+            negf_f4 c2 = z4;
 #pragma unroll
             for (int qq = 0; qq < 4; ++qq) {
                 const cfl av = s.Pr[BUF][(4 * qq + k) * SH::LDR + 16 * g + ci];
                 const float bx = Pre[tt][SB][qq], by = Pim[tt][SB][qq];
-                ar = G::mfma(av.x, bx, ar);
-                ar = G::mfma(-av.y, by, ar);
-                ai = G::mfma(av.x, by, ai);
-                ai = G::mfma(av.y, bx, ai);
+                c1 = G::mfma(av.x, bx, c1);
+                c2 = G::mfma(av.y, by, c2);
+                c3 = G::mfma(av.x + av.y, bx + by, c3);
             }
-            Pre[tt][g] = ar;
-            Pim[tt][g] = ai;
+            Pre[tt][g] = c1 - c2;
+            Pim[tt][g] = c3 - c1 - c2;
         }
     }
 }
