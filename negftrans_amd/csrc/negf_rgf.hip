@@ -378,18 +378,17 @@ __device__ __forceinline__ void gen_matvec(const cfl *Zg, const cfl *cL, int Np,
 //     c_m = E_1 delta_{m,1} - X_{m-1} - D_m X_m - X_{m+1} - y_{m+1}     (FP64, rounded)
 // Row i belongs to the lane pair t = 2 i, 2 i + 1; lane h = t & 1 takes the columns
 // j = h, h + 2, ... of D_m (each D entry is built once) for all NB right-hand sides, and
-// the two partial sums meet through a DPP lane swap.
+// the two partial sums meet through a DPP lane swap.  The slice's V-hat window is read
+// through vs (LDS when staged) with the basis keys ckey, four columns per round.
 template <int NB>
 __device__ __forceinline__ void resid_scatter_nb(const SysView &sv, const JobView &jv, const int m, const cd *X,
                                                  const cfl *Y, const int ldb, const int nR, const int *Rc,
-                                                 const int *sg, SolveShared &so) {
+                                                 const int *sg, const cd *vs, const int *ckey, SolveShared &so) {
     const int Np = sv.Np, N = sv.N, Nz = sv.Nz;
     const int t = threadIdx.x, i = t >> 1, h = t & 1;
     const int slice = m - 1;
-    const int W = 2 * sv.NGX + 1;
-    const cd *__restrict__ vs = sv.vrec + (size_t)slice * W * (2 * sv.NGY + 1);
     const bool rin = i < N;
-    const int nxr = rin ? sv.nx[i] : 0, nyr = rin ? sv.ny[i] : 0;
+    const int rb = rin ? ckey[i] + sv.NGX + (2 * sv.NGX + 1) * sv.NGY : 0;
     const double kin = rin ? jv.kinh[i] : 0.0;
     cd dg = cmk(0.0, 0.0);
     if (rin) {
@@ -408,13 +407,21 @@ __device__ __forceinline__ void resid_scatter_nb(const SysView &sv, const JobVie
         __syncthreads();
         if (rin) {
             const int jn = min(64, N - j0);
-            for (int jj = h; jj < jn; jj += 2) {
-                const int col = j0 + jj;
-                const int ix = (nxr - sv.nx[col] + sv.NGX) + W * (nyr - sv.ny[col] + sv.NGY);
-                const cd d = d_entry(vs[ix], i == col, kin, jv.z, sv.scale, dg);
-                const cd *xr = so.xL + jj * NEGF_NRH;
+            // four columns per round: their window gathers are issued together
+            for (int jj0 = h; jj0 < jn; jj0 += 8) {
+                cd d[4];
 #pragma unroll
-                for (int b = 0; b < NB; ++b) acc[b] = cfma(d, xr[b], acc[b]);
+                for (int u = 0; u < 4; ++u) {
+                    const int jj = jj0 + 2 * u, col = j0 + jj;
+                    d[u] = jj < jn ? d_entry(vs[rb - ckey[col]], i == col, kin, jv.z, sv.scale, dg) : cmk(0.0, 0.0);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int jj = min(jj0 + 2 * u, 63);
+                    const cd *xr = so.xL + jj * NEGF_NRH;
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) acc[b] = cfma(d[u], xr[b], acc[b]);
+                }
             }
         }
         __syncthreads();
@@ -448,14 +455,14 @@ __device__ __forceinline__ void resid_scatter_nb(const SysView &sv, const JobVie
 
 __device__ __forceinline__ void resid_scatter(const SysView &sv, const JobView &jv, const int m, const cd *X,
                                               const cfl *Y, const int ldb, const int nR, const int *Rc,
-                                              const int *sg, SolveShared &so) {
-    if (nR <= 2) resid_scatter_nb<2>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
-    else if (nR <= 4) resid_scatter_nb<4>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
-    else if (nR <= 6) resid_scatter_nb<6>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
-    else if (nR <= 8) resid_scatter_nb<8>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
-    else if (nR <= 10) resid_scatter_nb<10>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
-    else if (nR <= 12) resid_scatter_nb<12>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
-    else resid_scatter_nb<16>(sv, jv, m, X, Y, ldb, nR, Rc, sg, so);
+                                              const int *sg, const cd *vs, const int *ckey, SolveShared &so) {
+    if (nR <= 2) resid_scatter_nb<2>(sv, jv, m, X, Y, ldb, nR, Rc, sg, vs, ckey, so);
+    else if (nR <= 4) resid_scatter_nb<4>(sv, jv, m, X, Y, ldb, nR, Rc, sg, vs, ckey, so);
+    else if (nR <= 6) resid_scatter_nb<6>(sv, jv, m, X, Y, ldb, nR, Rc, sg, vs, ckey, so);
+    else if (nR <= 8) resid_scatter_nb<8>(sv, jv, m, X, Y, ldb, nR, Rc, sg, vs, ckey, so);
+    else if (nR <= 10) resid_scatter_nb<10>(sv, jv, m, X, Y, ldb, nR, Rc, sg, vs, ckey, so);
+    else if (nR <= 12) resid_scatter_nb<12>(sv, jv, m, X, Y, ldb, nR, Rc, sg, vs, ckey, so);
+    else resid_scatter_nb<16>(sv, jv, m, X, Y, ldb, nR, Rc, sg, vs, ckey, so);
 }
 
 // cL[sigma(i) 16 + b] = coef * V(i, b) (b < nR, else 0) for a row-major Np x ldb array V.
@@ -483,9 +490,9 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
     const size_t NN = (size_t)Np * Np;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int jj = lane & 15, q = lane >> 4;
-    // basis keys for the V-hat window index of the block build, and the slice's window
-    // staged in LDS when it fits (the Gauss-Jordan's staging area, free outside the
-    // inversion); otherwise read from HBM
+    // basis keys for the V-hat window index of the block build and the residual, and the
+    // slice's window staged in LDS when it fits (the Gauss-Jordan's staging area, free
+    // outside the inversion); otherwise read from HBM
     const int nvh = (2 * sv.NGX + 1) * (2 * sv.NGY + 1);
     const bool vh_lds = nvh <= BgjeShared<>::VHCAP;
     for (int c = t; c < Np; c += blockDim.x) ckey[c] = c < sv.N ? sv.nx[c] + (2 * sv.NGX + 1) * sv.ny[c] : 0;
@@ -579,7 +586,9 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
         double dprev = -1.0;
         for (int it = 0; it < ma.nref && !conv; ++it) {
             for (int m = Nz; m >= 1; --m) {           // y_m = g_m (r_m - y_{m+1})
-                resid_scatter(sv, jv, m, X, Y, ldb, nR, Rc, hsig + (size_t)(m - 1) * Np, so);
+                stage_vh(m - 1);
+                __syncthreads();
+                resid_scatter(sv, jv, m, X, Y, ldb, nR, Rc, hsig + (size_t)(m - 1) * Np, vsrc(m - 1), ckey, so);
                 __syncthreads();
                 negf_f4 ore[2], oim[2];
                 gen_matvec(hist + (size_t)(m - 1) * NN, so.cL, Np, ore, oim);
