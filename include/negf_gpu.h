@@ -37,6 +37,17 @@ typedef struct negf_ctx negf_ctx;
  * channel does not converge).  Trace weights become Im(term) / Delta^2 on the
  * lattice-open channels.  A V = 0 device then transmits exactly 1 per open channel. */
 #define NEGF_LEAD_DECIMATION 32
+/* Extension, no reference counterpart (the reference's only tuning knob is LAPACK's
+ * ilaenv block size, src/math_kernel.f90:17): FP64 negf_transmission_batch picks its
+ * kernel shape and form by measurement.  Variants: 0 one workgroup per job with the
+ * 32-column two-tile Gauss-Jordan (Np <= 512), 1 two workgroups per job (two-sided) with
+ * it, 2 one workgroup per job with the 16-column four-tile form, 3 two-sided with it
+ * (above Np = 512 only 2 and 3).  On the first call of a job-count class (ceil(log2
+ * jobs)) every variant runs over the call's own jobs; the fastest is kept for the class
+ * and its results are returned.  Without the flag a fixed rule picks: two-sided when two
+ * workgroups per job fit the GPU at once, two-tile up to Np = 512.  The environment
+ * variable NEGF_TRANS_VARIANT (read at context creation) forces one variant. */
+#define NEGF_AUTOTUNE        64
 
 /* Library version string and last error message (thread-local). */
 const char *negf_version(void);
@@ -202,6 +213,12 @@ double negf_last_kernel_seconds(negf_ctx *ctx);
  * not reach its tolerance within the iteration limit and were recomputed on the FP64
  * path (their results are FP64 results).  0 for FP64 contexts. */
 int negf_last_fallbacks(const negf_ctx *ctx);
+
+/* Extension, no reference counterpart.  Variant (see NEGF_AUTOTUNE) of the last FP64
+ * negf_transmission_batch call, -1 before any; and the device seconds each variant took
+ * in the context's last tuning call (-1 for a variant not run). */
+int negf_last_variant(const negf_ctx *ctx);
+double negf_tune_seconds(const negf_ctx *ctx, int variant);
 
 /*
  * NonLocalPotential_RealToPlanewave (src/grid.f90:137-234; the reference never calls it):

@@ -43,6 +43,8 @@ SIGNATURES = {
     "negf_matrix_product_batch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
     "negf_last_kernel_seconds": (ctypes.c_double, [_vp]),
     "negf_last_fallbacks": (ctypes.c_int, [_vp]),
+    "negf_last_variant": (ctypes.c_int, [_vp]),
+    "negf_tune_seconds": (ctypes.c_double, [_vp, ctypes.c_int]),
     "negf_ctx_set_potential": (ctypes.c_int, [_vp, _dp]),
     "negf_multi_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]),
     "negf_multi_transmission": (ctypes.c_int, [_vp, ctypes.POINTER(_dp), _ip, ctypes.POINTER(_dp),

@@ -1557,6 +1557,12 @@ struct negf_ctx {
     bool diag_one_sided = false; // transmission, D mode, bias window: never the two-sided forms (NEGF_DIAG_ONE_SIDED=1)
     bool mixed_fallback = true;  // mixed path: recompute unconverged energies in FP64 (NEGF_MIXED_FALLBACK=0 keeps
                                  // the unconverged mixed values, for measuring the refinement)
+    // NEGF_AUTOTUNE: FP64 transmission variant per job-count class (ceil(log2 jobs)),
+    // chosen by timing every variant on the first call of the class
+    std::map<int, int> tuned;
+    double tune_s[4] = {-1.0, -1.0, -1.0, -1.0};   // device seconds of each variant in the last tuning call
+    int last_variant = -1;                          // variant of the last FP64 transmission call
+    int force_variant = -1;                         // NEGF_TRANS_VARIANT: one variant for every call
 };
 
 namespace {
@@ -1803,6 +1809,31 @@ static void launch_trans(int Np, int slots, int threads, hipStream_t st, const T
     else hipLaunchKernelGGL((k_transmission<1024, 16, true, MULTI, NEGF_MAXNP>), dim3(slots), dim3(threads), 0, st, a);
 }
 
+// FP64 transmission variants: 0 one workgroup per job with the 32-column two-tile
+// Gauss-Jordan (Np <= 512), 1 two workgroups per job (two-sided) with it, 2 one workgroup
+// per job with the 16-column four-tile form (ceil(Np / 64) waves, any Np), 3 two-sided with
+// it.  Above Np = 512 only 2 and 3 exist.
+static bool tvar_four(int v) { return v >= 2; }
+static bool tvar_split(int v) { return v & 1; }
+static int tvar_threads(int Np, int v) {
+    return tvar_four(v) ? std::min(1024, round_up(Np, 64)) : threads_for_shape(Np);
+}
+static const void *tvar_kernel(int Np, int v) {
+    if (!tvar_four(v)) return tvar_split(v) ? trans_split_kernel(Np) : trans_kernel<false>(Np);
+    return tvar_split(v) ? (const void *)k_transmission_split<1024, 16, true, NEGF_MAXNP>
+                            : (const void *)k_transmission<1024, 16, true, false, NEGF_MAXNP>;
+}
+static void launch_tvar(int Np, int v, int grid, int threads, hipStream_t st, const TransArgs &a) {
+    if (!tvar_four(v)) {
+        if (tvar_split(v)) launch_trans_split(Np, grid, threads, st, a);
+        else launch_trans<false>(Np, grid, threads, st, a);
+    } else if (tvar_split(v)) {
+        hipLaunchKernelGGL((k_transmission_split<1024, 16, true, NEGF_MAXNP>), dim3(grid), dim3(threads), 0, st, a);
+    } else {
+        hipLaunchKernelGGL((k_transmission<1024, 16, true, false, NEGF_MAXNP>), dim3(grid), dim3(threads), 0, st, a);
+    }
+}
+
 static void launch_inverse(int Np, int count, hipStream_t st, int N, const cd *A, const cd *Sub, cd *work,
                            cd *inv, int *status) {
     const int th = threads_for_shape(Np);
@@ -1824,6 +1855,10 @@ const char *negf_version(void) { return "negf_gpu 0.1 (gfx950, FP64 RGF)"; }
 const char *negf_last_error(void) { return g_err.c_str(); }
 
 int negf_last_fallbacks(const negf_ctx *c) { return c ? c->last_fallbacks : 0; }
+int negf_last_variant(const negf_ctx *c) { return c ? c->last_variant : -1; }
+double negf_tune_seconds(const negf_ctx *c, int variant) {
+    return (c && variant >= 0 && variant < 4) ? c->tune_s[variant] : -1.0;
+}
 
 int negf_device_count(void) {
     int n = 0;
@@ -1859,6 +1894,7 @@ int negf_ctx_create(negf_ctx **out, int N, int N_z, const int *nx_grid, const in
     if (const char *r = std::getenv("NEGF_MIXED_REFINE")) c->mixed_refine = std::max(0, std::atoi(r));
     if (const char *r = std::getenv("NEGF_MIXED_TOL")) c->mixed_tol = std::max(1e-15, std::atof(r));
     if (const char *r = std::getenv("NEGF_DIAG_ONE_SIDED")) c->diag_one_sided = std::atoi(r) != 0;
+    if (const char *r = std::getenv("NEGF_TRANS_VARIANT")) c->force_variant = std::atoi(r);
     if (const char *r = std::getenv("NEGF_MIXED_FALLBACK")) c->mixed_fallback = std::atoi(r) != 0;
     c->Lx = Lx;
     c->Ly = Ly;
@@ -2135,23 +2171,32 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
     a.nLmax = nLmax;
     a.tau = d_tau;
     a.status = d_st;
-    // Few jobs (at most half the workgroups the GPU holds at once): the two-sided form,
-    // two workgroups per job in the first launch, one in the second
-    {
-        const void *kfs = trans_split_kernel(c->Np);
+    // The variant by the fixed rule: the two-sided form when two workgroups per job fit
+    // the GPU at once (few jobs), the two-tile Gauss-Jordan up to Np = 512.
+    const bool big = c->Np > 512;
+    auto split_fits = [&](int v) {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfs, threads, 0) != hipSuccess || per_cu < 1)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tvar_kernel(c->Np, v), tvar_threads(c->Np, v),
+                                                         0) != hipSuccess || per_cu < 1)
             per_cu = 1;
-        if (c->Nz >= 2 && 2L * nj <= (long)c->n_cu * per_cu && !c->diag_one_sided) {
+        return 2L * nj <= (long)c->n_cu * per_cu;
+    };
+    int rule = big ? 2 : 0;
+    if (c->Nz >= 2 && !c->diag_one_sided && split_fits(rule + 1)) rule += 1;
+    // Launches variant v over all jobs between the context's events.
+    auto run_variant = [&](int v) -> int {
+        const int th = tvar_threads(c->Np, v);
+        const int pw = std::max(panel_width(c->Np), tvar_four(v) ? 16 : 32);
+        if (tvar_split(v)) {
             int ldX = 1;
             for (int j = 0; j < nj; ++j) ldX = std::max(ldX, std::max(nL[j], nR[j]));
-            const size_t NN = (size_t)c->Np * c->Np, XX = (size_t)ldX * c->Np,
-                         SCR = (size_t)2 * panel_width(c->Np) * c->Np;
+            const size_t NN = (size_t)c->Np * c->Np, XX = (size_t)ldX * c->Np, SCR = (size_t)2 * pw * c->Np;
             const size_t per_job = (2 * (2 * NN + 2 * XX + SCR) + 3 * NN + XX + SCR) * sizeof(cd);
             int *d_st2;
-            if ((st = tmp.alloc(&d_st2, 2 * (size_t)nj))) return st;
-            const int P = slots_for(c, kfs, threads, nj, per_job);
-            if ((st = ensure_ws(c, per_job * P))) return st;
+            int e;
+            if ((e = tmp.alloc(&d_st2, 2 * (size_t)nj))) return e;
+            const int P = slots_for(c, tvar_kernel(c->Np, v), th, nj, per_job);
+            if ((e = ensure_ws(c, per_job * P))) return e;
             a.ws = (cd *)c->ws;
             a.ldX = ldX;
             a.status2 = d_st2;
@@ -2161,39 +2206,76 @@ int negf_transmission_batch(negf_ctx *c, const double *E, int nE, const double *
                 const int n = std::min(P, nj - base);
                 for (int ph = 0; ph < 2; ++ph) {
                     a.phase = ph;
-                    launch_trans_split(c->Np, ph == 0 ? 2 * n : n, threads, c->stream, a);
+                    launch_tvar(c->Np, v, ph == 0 ? 2 * n : n, th, c->stream, a);
                     HIPCHK(hipGetLastError());
                 }
             }
             HIPCHK(hipEventRecord(c->ev1, c->stream));
-            return finish_transmission(c, nj, d_tau, d_st, je, jk, kw, tau, tau_jobs);
+            return 0;
+        }
+        const size_t per_slot = ((size_t)2 * c->Np * c->Np + (size_t)2 * nLmax * c->Np + (size_t)2 * pw * c->Np) * sizeof(cd);
+        const int slots = slots_for(c, tvar_kernel(c->Np, v), th, nj, per_slot);
+        int e;
+        if ((e = ensure_ws(c, per_slot * slots))) return e;
+        a.ws = (cd *)c->ws;
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        launch_tvar(c->Np, v, slots, th, c->stream, a);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        return 0;
+    };
+    int variant = rule;
+    if (c->force_variant >= 0) {
+        if (c->force_variant > 3 || (big && c->force_variant < 2) || (tvar_split(c->force_variant) && c->Nz < 2))
+            return fail(-2, "NEGF_TRANS_VARIANT " + std::to_string(c->force_variant) + " does not exist for this size");
+        variant = c->force_variant;
+    } else if ((c->flags & NEGF_AUTOTUNE) && !c->diag_one_sided) {
+        // Job-count class ceil(log2 jobs): on its first call every variant that exists
+        // for this Np runs over the call's own jobs, the fastest (HIP events) is kept for
+        // the class, and its results are returned.
+        int cls = 0;
+        while ((1 << cls) < nj) ++cls;
+        auto it = c->tuned.find(cls);
+        if (it != c->tuned.end()) {
+            variant = it->second;
+        } else {
+            std::vector<double> best_tau(nj);
+            std::vector<int> best_st(nj);
+            float best_ms = 0.f;
+            int best = -1;
+            for (int v = 0; v < 4; ++v) c->tune_s[v] = -1.0;
+            for (int v = big ? 2 : 0; v < 4; ++v) {
+                if (tvar_split(v) && c->Nz < 2) continue;
+                if ((st = run_variant(v))) return st;
+                std::vector<double> h_tau(nj);
+                std::vector<int> h_st(nj);
+                HIPCHK(hipMemcpyAsync(h_tau.data(), d_tau, nj * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipMemcpyAsync(h_st.data(), d_st, nj * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+                c->tune_s[v] = ms * 1e-3;
+                if (best < 0 || ms < best_ms) {
+                    best = v;
+                    best_ms = ms;
+                    best_tau.swap(h_tau);
+                    best_st.swap(h_st);
+                }
+            }
+            c->tuned[cls] = best;
+            c->last_variant = best;
+            c->last_kernel_s = best_ms * 1e-3;
+            int status = 0;
+            for (int j = 0; j < nj; ++j) {
+                tau[je[j]] += best_tau[j] * kw[jk[j]];
+                if (tau_jobs) tau_jobs[j] = best_tau[j];
+                if (best_st[j] && !status) status = best_st[j];
+            }
+            return status;
         }
     }
-    const size_t per_slot =
-        ((size_t)2 * c->Np * c->Np + (size_t)2 * nLmax * c->Np + (size_t)2 * panel_width(c->Np) * c->Np) * sizeof(cd);
-    const void *kfn = trans_kernel<false>(c->Np);
-    const int slots = slots_for(c, kfn, threads, nj, per_slot);
-    if ((st = ensure_ws(c, per_slot * slots))) return st;
-    a.z = d_z;
-    a.kidx = d_k;
-    a.kinh = d_kin;
-    a.sigL = d_sl;
-    a.sigR = d_sr;
-    a.open_off = d_off;
-    a.nL = d_nL;
-    a.nR = d_nR;
-    a.Lrow = d_Lr;
-    a.Rcol = d_Rc;
-    a.gL = d_gl;
-    a.gR = d_gr;
-    a.nLmax = nLmax;
-    a.ws = (cd *)c->ws;
-    a.tau = d_tau;
-    a.status = d_st;
-    HIPCHK(hipEventRecord(c->ev0, c->stream));
-    launch_trans<false>(c->Np, slots, threads, c->stream, a);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    if ((st = run_variant(variant))) return st;
+    c->last_variant = variant;
     return finish_transmission(c, nj, d_tau, d_st, je, jk, kw, tau, tau_jobs);
 }
 

@@ -27,6 +27,7 @@ MODE_D, MODE_F, MODE_L = 1, 2, 4
 PREC_FP64, PREC_MIXED_FP32 = 0, 1
 FULL_WINDOW = 16
 LEAD_DECIMATION = 32     # extension: lattice lead terms by decimation on the GPU
+AUTOTUNE = 64            # extension: FP64 transmission kernel shape and form picked by timing
 
 
 # ------------------------------------------------------------------ parameters
@@ -359,6 +360,18 @@ class NegfDevice:
     def last_fallbacks(self):
         """Energies of the last mixed-precision transmission call recomputed in FP64."""
         return self._lib.negf_last_fallbacks(self._h)
+
+    @property
+    def last_variant(self):
+        """Kernel variant of the last FP64 transmission call (0 one-sided two-tile, 1
+        two-sided two-tile, 2 one-sided four-tile, 3 two-sided four-tile)."""
+        return self._lib.negf_last_variant(self._h)
+
+    def tune_seconds(self):
+        """Device seconds of each variant in the context's last tuning call (AUTOTUNE),
+        None for a variant that was not run."""
+        out = [self._lib.negf_tune_seconds(self._h, v) for v in range(4)]
+        return [x if x >= 0 else None for x in out]
 
     def set_potential(self, vrec):
         """Replace V_reciprocal_all (same window and N_z), e.g. in a self-consistent loop."""
