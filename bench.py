@@ -8,8 +8,9 @@ Workload: the disordered-wire transmission configuration N_pw = 317, N_z = 1000
 seeded), energies from the uniform 0-30 meV grid of 4096 points.  A step is one
 batched launch of the transmission sweep over a strided quarter-or-less of the grid: B
 energies on the FP64 path (default 256, one job per resident workgroup slot) and B'
-on the mixed path (FP32 sweep + FP64 refinement; default 1024, four jobs per slot, taken
-in turn).  Inputs (V-hat table, basis) are resident in HBM before the timed region.
+on the mixed path (FP32 sweep + FP64 refinement; default 2048, eight jobs per slot, taken
+in turn: the launch ends when the last slot's last job does, and with eight jobs per
+slot that last round is a smaller share of the launch than with four).  Inputs (V-hat table, basis) are resident in HBM before the timed region.
 `value` is the faster path among those that pass the gate max |T - T_FP64| <= 1e-6
 (checked on the mixed path's last step), and "precisions" lists both.  Prints one
 JSON line.
@@ -113,8 +114,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0,
                     help="energies per FP64 step (default 256, one job per resident workgroup)")
     ap.add_argument("--batch-mixed", type=int, default=0,
-                    help="energies per mixed step (default 1024: workgroups take jobs in turn, so "
-                         "four per slot spread the jobs that need more refinement)")
+                    help="energies per mixed step (default 2048: workgroups take jobs in turn, eight "
+                         "per slot)")
     ap.add_argument("--config", choices=("disorder_c1", "qpc_c0"), default="disorder_c1",
                     help="qpc_c0: only the N = 61, N_z = 200 leg on the GPU (for profiling)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -154,11 +155,12 @@ def main():
     nx, ny = wl.basis()
     vrec = negf.local_potential_real_to_planewave(wl.V_real_eV / negf.HARTREE, NGX, NGY)
     Eall = wl.energies_hartree()
-    batches = {"fp64": args.batch or 256, "mixed": args.batch_mixed or 1024}
+    batches = {"fp64": args.batch or 256, "mixed": args.batch_mixed or 2048}
 
     def energies_for(step, batch):
-        # a strided sweep of the 4096-point grid, a different offset every step
-        idx = (np.arange(batch) * (len(Eall) // batch) + step) % len(Eall)
+        # a strided sweep of the 4096-point grid (spread over all of it for any batch), a
+        # different offset every step
+        idx = (np.arange(batch) * len(Eall) // batch + step) % len(Eall)
         return Eall[idx]
 
     def run_path(flags, label):
