@@ -31,17 +31,17 @@ PEAK_FP32_TFLOPS = 157.3     # MI355X dense FP32 matrix
 GATE = 1e-6                  # max |T_mixed - T_FP64| for the mixed path to count
 N_PW, N_Z = 317, 1000
 # HBM bytes per energy point of the dominant kernel: rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE over one launch (256 FP64 / 1024 mixed energies), scaled by a 1 GiB copy's counts (FETCH x2.0,
+# WRITE_SIZE over one launch (256 FP64 / 2048 mixed energies), scaled by a 1 GiB copy's counts (FETCH x2.0,
 # WRITE x1.0); the command lines and raw counts are in profiles/.
-TRAFFIC_BYTES_PER_POINT = {"fp64": 4.298e10, "mixed": 1.675e10}
+TRAFFIC_BYTES_PER_POINT = {"fp64": 4.298e10, "mixed": 1.726e10}
 # Matrix-core FLOPs executed per energy point.  FP64 kernel: the Gauss-Jordan inverse of
 # every slice (8 Np^3 at the padded size Np = 320) plus the open-row product on 16-row
 # tiles (8 * 16 * Np^2); SQ_INSTS_VALU_MFMA_MOPS_F64 reads 2.48e11 per point.  Mixed
 # kernel: FP32 matrix-core work of the sweep (five 3M trailing passes and the in-panel
 # sub-block products) and of the generator products of the first solve and of the
-# refinement; SQ_INSTS_VALU_MFMA_MOPS_F32 reads 2.89e11 per point (its FP64 residual runs
+# refinement; SQ_INSTS_VALU_MFMA_MOPS_F32 reads 2.55e11 per point (its FP64 residual runs
 # on the VALU and is not counted).
-EXECUTED_FLOP_PER_POINT = {"fp64": N_Z * (8.0 * 320 ** 3 + 8.0 * 16 * 320 ** 2), "mixed": 2.891e11}
+EXECUTED_FLOP_PER_POINT = {"fp64": N_Z * (8.0 * 320 ** 3 + 8.0 * 16 * 320 ** 2), "mixed": 2.553e11}
 
 
 def flops_per_point(N, Nz):
