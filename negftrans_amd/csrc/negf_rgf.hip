@@ -484,6 +484,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
     __shared__ BgjeShared<> s;
     __shared__ SolveShared so;
     __shared__ int ckey[NEGF_MAXNP4];
+    static_assert(sizeof(so.cL) >= BgjeShared<>::TSZ * sizeof(cfl), "second staging buffer");
     const TransArgs &a = ma.a;
     const SysView &sv = a.sv;
     const int Np = sv.Np, Nz = sv.Nz, ldb = ma.ldb;
@@ -544,7 +545,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
             __syncthreads();
             build_block_keys(M, sv, jv, m - 1, m == Nz ? nullptr : hist + (size_t)m * NN, vsrc(m - 1), ckey);
             __syncthreads();
-            if (bgje_invert64(M, Np, s)) grew = true;
+            if (bgje_invert64(M, Np, s, so.cL)) grew = true;    // the solve's staging area is free here
             for (int i = t; i < Np; i += blockDim.x) hsig[(size_t)(m - 1) * Np + i] = i;
         }
         __syncthreads();

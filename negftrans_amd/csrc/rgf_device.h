@@ -1041,13 +1041,16 @@ __device__ __forceinline__ int gje_invert4(typename GjePrec<R>::C *__restrict__ 
 
 template <int MAXNP = NEGF_MAXNP4>
 struct BgjeShared {
-    static constexpr int CH = 64, LDT = CH + 1;     // staged old pivot rows (64 x CH), padded stride
+    // staged old pivot rows of one chunk of CH columns, column-major: Tl[cl * LDT + p]
+    // (p < 64 the pivot row in the panel); the stride is padded by one complex value
+    static constexpr int CH = 64, LDT = 64 + 1;
     static constexpr int LDP = 17, LDR = 65;        // 16 x 16 diagonal block, 16 pivot rows of the panel
     static constexpr int MAXW = 2 * MAXNP / 64;
-    static constexpr int VHCAP = (64 * LDT * 8 + 2 * 16 * LDP * 16 + 2 * 16 * LDR * 8) / 16;
+    static constexpr int TSZ = CH * LDT;            // complex values in one staging buffer
+    static constexpr int VHCAP = (TSZ * 8 + 2 * 16 * LDP * 16 + 2 * 16 * LDR * 8) / 16;
     union {
         struct {
-            cfl Tl[64 * LDT];
+            cfl Tl[TSZ];
             cd Pd[2][16 * LDP];  // M_{K_s K_s}, inverted in place in FP64 (alternating by sub-block)
             cfl Pr[2][16 * LDR]; // the sub-block's pivot rows over the panel columns, before the step
         };
@@ -1091,11 +1094,18 @@ __device__ __forceinline__ void bgje_diag16(cd *A, bool &bad) {
 }
 
 // One sub-block step SB (static) of bgje_invert64's panel: see the comment above.
-template <int SB, class SH>
+// after_sync() runs right after the step's barrier.  The lane indices are derived here
+// from an opaque copy of the thread index: kept live across the panel instead, they and
+// the LDS addresses built from them were spilled, and every step waited for the reloads.
+template <int SB, class SH, class F>
 __device__ __forceinline__ void bgje_substep(negf_f4 (&Pre)[2][4], negf_f4 (&Pim)[2][4], const int k0, const int nsb,
-                                             const int w, const int j, const int k, const int ci, SH &s, bool &bad) {
+                                             const int w, SH &s, bool &bad, F &&after_sync) {
     constexpr int TS = SB & 1, BUF = SB & 1;
     typedef GjePrec<float> G;
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
+    const int j = lane & 15, k = lane >> 4;
+    const int ci = ((lane & 3) << 2) | ((lane >> 2) & 3);      // A row order c(i), i = lane & 15
     const bool own = (w == (k0 >> 5) + (SB >> 1));      // wave-uniform
     if (own) {
         // publish the diagonal block in FP64 and the pivot rows over the panel, then invert
@@ -1108,6 +1118,7 @@ __device__ __forceinline__ void bgje_substep(negf_f4 (&Pre)[2][4], negf_f4 (&Pim
         bgje_diag16(s.Pd[BUF], bad);
     }
     __syncthreads();
+    after_sync();
     // W = -M_{I K_s} P for the tiles outside K_s, W = P for the owner tile
 #pragma unroll
     for (int tt = 0; tt < 2; ++tt) {
@@ -1172,14 +1183,15 @@ __device__ __forceinline__ void bgje_substep(negf_f4 (&Pre)[2][4], negf_f4 (&Pim
 // panel when Np % 64 == 32.  The panel width is a template argument so that every panel
 // register is indexed statically.
 template <int NSB, class SH>
-__device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, const int k0, SH &s, bool &bad) {
+__device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, const int k0, SH &s, cfl *T2,
+                                           bool &bad) {
     typedef GjePrec<float> G;
     typedef negf_f4 Acc;
     constexpr int KW = 16 * NSB;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int t = threadIdx.x, lane = t & 63, nw = blockDim.x >> 6;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int j = lane & 15, k = lane >> 4;
     const int R0 = w << 5;
-    const int ci = ((lane & 3) << 2) | ((lane >> 2) & 3);      // A row order c(i), i = lane & 15
     const __amdgpu_buffer_rsrc_t zr = negf_rsrc(Z, (unsigned)(Np * Np) * 8u);
     const unsigned cstr = (unsigned)Np * 8u;                   // one column
     auto voff = [&](int row, int col) { return (unsigned)(row + col * Np) * 8u; };
@@ -1201,11 +1213,38 @@ __device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, co
                 Pre[tt][g][r] = v.x;
                 Pim[tt][g][r] = v.y;
             }
-    bgje_substep<0>(Pre, Pim, k0, NSB, w, j, k, ci, s, bad);
-    bgje_substep<1>(Pre, Pim, k0, NSB, w, j, k, ci, s, bad);
+    // The trailing update's column chunks, skipping the panel's own.  The old pivot rows
+    // Z_{K J} of a chunk go into LDS by DMA (buffer_load ... lds: one wave instruction
+    // moves 64 floats, 32 pivot rows of one column, without passing through registers), so
+    // a chunk's copy runs behind the panel or the previous chunk's products.  Those rows
+    // keep their old values until the chunk's own update: the panel steps touch only the
+    // panel columns.
+    const int cfirst = (k0 == 0) ? SH::CH : 0;
+    auto next_chunk = [&](int c0) { return (c0 + SH::CH == k0) ? k0 + SH::CH : c0 + SH::CH; };
+    auto stage = [&](cfl *buf, const int c0) {
+        constexpr int nh = KW / 32;
+        const int ninst = (min(c0 + SH::CH, Np) - c0) * nh;
+        for (int n = w; n < ninst; n += nw) {
+            const int cl = n / nh, h = n - cl * nh;      // wave-uniform
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(zr, buf + cl * SH::LDT + 32 * h, 4,
+                                                     voff(k0 + 32 * h, c0 + cl) + 4u * (unsigned)lane, 0, 0, 0);
+        }
+    };
+    // the first chunk's DMA goes out after the last sub-block's barrier: issued earlier,
+    // it would hold up the sub-block barriers (a barrier waits for the wave's memory
+    // operations)
+    auto none = [] {};
+    auto first = [&] {
+        if (cfirst < Np) stage(s.Tl, cfirst);
+    };
     if constexpr (NSB == 4) {
-        bgje_substep<2>(Pre, Pim, k0, NSB, w, j, k, ci, s, bad);
-        bgje_substep<3>(Pre, Pim, k0, NSB, w, j, k, ci, s, bad);
+        bgje_substep<0>(Pre, Pim, k0, NSB, w, s, bad, none);
+        bgje_substep<1>(Pre, Pim, k0, NSB, w, s, bad, none);
+        bgje_substep<2>(Pre, Pim, k0, NSB, w, s, bad, none);
+        bgje_substep<3>(Pre, Pim, k0, NSB, w, s, bad, first);
+    } else {
+        bgje_substep<0>(Pre, Pim, k0, NSB, w, s, bad, none);
+        bgje_substep<1>(Pre, Pim, k0, NSB, w, s, bad, first);
     }
     // the panel columns take their final values
 #pragma unroll
@@ -1215,9 +1254,8 @@ __device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, co
 #pragma unroll
             for (int r = 0; r < 4; ++r)
                 G::store(cmk(Pre[tt][g][r], Pim[tt][g][r]), zr, voff(R0 + 16 * tt + j, k0 + 16 * g + 4 * r + k), 0u);
-    // trailing update, one chunk of CH columns at a time: the old pivot rows Z_{K J} are
-    // staged from Z (their new values are written only by the chunk's own update, after
-    // the staging barrier); rows in K start from zero
+    // trailing update, one chunk of CH columns at a time from the staged old pivot rows;
+    // rows in K start from zero
     const bool zb = (R0 >= k0 && R0 < k0 + KW);
     auto next_tile = [&](int ct) {
         const int n = (ct + 16 == k0) ? k0 + 64 : ct + 16;
@@ -1236,14 +1274,17 @@ __device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, co
         }
     };
     load_base(k0 == 0 ? 64 : 0);
-    for (int c0 = 0; c0 < Np; c0 += SH::CH) {
-        if (c0 == k0) continue;                              // the panel's own columns
-        const int c1 = min(c0 + SH::CH, Np);
-        for (int e = t; e < (c1 - c0) * KW; e += blockDim.x) {
-            const int pp = e % KW, cl = e / KW;
-            s.Tl[pp * SH::LDT + cl] = Z[(k0 + pp) + (size_t)(c0 + cl) * Np];
-        }
+    // One chunk from the buffer tb while the next one (from cn, if any) is staged into tn.
+    // The barrier first: it waits for this wave's DMA of the chunk (a barrier waits for the
+    // wave's memory operations) and for every wave to be done with tn (the chunk before).
+    // The next chunk's DMA is issued in the last tile, after that tile's base prefetch:
+    // the memory counter is in order, so a DMA issued earlier would hold up the waits for
+    // the tile bases in between.  The two buffers alternate in a loop unrolled by two, so
+    // that each LDS read names its buffer statically.
+    auto run_chunk = [&](const cfl *tb, cfl *tn, const int c0, const int cn) {
+        __builtin_amdgcn_s_waitcnt(0x0f70);             // vmcnt(0): this wave's DMA has landed
         __syncthreads();
+        const int c1 = min(c0 + SH::CH, Np);
         for (int ct = c0; ct < c1; ct += 16) {
             // 3M complex products: with A = T (staged pivot rows) and B = W (panel),
             //   c1 += Ar Br,  c2 += Ai Bi,  c3 += (Ar + Ai)(Br + Bi),
@@ -1261,13 +1302,14 @@ __device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, co
                 c21[r] = 0.f;
             }
             load_base(next_tile(ct));
+            if (ct + 16 >= c1 && cn < Np) stage(tn, cn);
             __builtin_amdgcn_sched_barrier(0);          // keep the prefetch ahead of the MFMAs
             int lx = threadIdx.x & 63;
             asm volatile("" : "+v"(lx));                 // recompute, do not keep live
-            const cfl *tl = s.Tl + (lx >> 4) * SH::LDT + (ct - c0 + (lx & 15));
+            const cfl *tl = tb + (ct - c0 + (lx & 15)) * SH::LDT + (lx >> 4);
 #pragma unroll
             for (int q = 0; q < 4 * NSB; ++q) {
-                const cfl tv = tl[4 * q * SH::LDT];
+                const cfl tv = tl[4 * q];
                 const float ts = tv.x + tv.y;
                 const float p0r = Pre[0][q >> 2][q & 3], p0i = Pim[0][q >> 2][q & 3];
                 const float p1r = Pre[1][q >> 2][q & 3], p1i = Pim[1][q >> 2][q & 3];
@@ -1285,21 +1327,31 @@ __device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, co
                 G::store(cmk(c11[r] - c21[r], c31[r] - c11[r] - c21[r]), zr, vo + 128u, r * cstr);
             }
         }
-        __syncthreads();
+    };
+    for (int c0 = cfirst; c0 < Np;) {
+        int cn = next_chunk(c0);
+        run_chunk(s.Tl, T2, c0, cn);
+        c0 = cn;
+        if (c0 >= Np) break;
+        cn = next_chunk(c0);
+        run_chunk(T2, s.Tl, c0, cn);
+        c0 = cn;
     }
+    __syncthreads();      // the chunks' writes before the next panel; the buffers are free
 }
 
 // In-place inversion of Z (Np x Np, Np % 32 == 0, blockDim.x == 2 Np); on return
 // inverse = Z and s.sigma is the identity.  Returns 0, or 1 if the growth check failed.
+// T2: a second LDS staging buffer of SH::TSZ complex values, free for the call.
 template <class SH>
-__device__ __forceinline__ int bgje_invert64(cfl *__restrict__ Z, const int Np, SH &s) {
+__device__ __forceinline__ int bgje_invert64(cfl *__restrict__ Z, const int Np, SH &s, cfl *T2) {
     const int t = threadIdx.x, lane = t & 63;
     for (int i = t; i < Np; i += blockDim.x) s.sigma[i] = i;
     if (t == 0) s.bad = 0;
     bool bad = false;
     for (int k0 = 0; k0 < Np; k0 += 64) {
-        if (k0 + 64 <= Np) bgje_panel<4>(Z, Np, k0, s, bad);
-        else bgje_panel<2>(Z, Np, k0, s, bad);
+        if (k0 + 64 <= Np) bgje_panel<4>(Z, Np, k0, s, T2, bad);
+        else bgje_panel<2>(Z, Np, k0, s, T2, bad);
     }
     if (__ballot(bad) && lane == 0) atomicOr(&s.bad, 1);
     __syncthreads();
