@@ -305,8 +305,13 @@ struct MixedArgs {
 };
 
 struct SolveShared {
-    cfl cL[NEGF_MAXNP4 * NEGF_NRH];   // one right-hand side, rows permuted: cL[sigma(i) 16 + b] = v(i, b)
-    cd xL[64 * NEGF_NRH];             // 64 rows of X_m for the residual
+    union {
+        struct {
+            cfl cL[NEGF_MAXNP4 * NEGF_NRH];   // one right-hand side, rows permuted: cL[sigma(i) 16 + b] = v(i, b)
+            cd xL[64 * NEGF_NRH];             // 64 rows of X_m for the residual
+        };
+        cfl stage[NEGF_MAXNP4 * NEGF_NRH + 2 * 64 * NEGF_NRH];   // the sweep's: free outside the solves
+    };
 };
 
 // out = g_m v for the 16 columns staged in cL, on the FP32 matrix cores.  Zg is the stored
@@ -484,7 +489,8 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
     __shared__ BgjeShared<> s;
     __shared__ SolveShared so;
     __shared__ int ckey[NEGF_MAXNP4];
-    static_assert(sizeof(so.cL) >= BgjeShared<>::TSZ * sizeof(cfl), "second staging buffer");
+    static_assert(sizeof(so.stage) >= (BgjeShared<>::TSZ + 2 * BgjeShared<>::WLSZ) * sizeof(cfl),
+                  "second staging buffer");
     const TransArgs &a = ma.a;
     const SysView &sv = a.sv;
     const int Np = sv.Np, Nz = sv.Nz, ldb = ma.ldb;
@@ -497,6 +503,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
     const int nvh = (2 * sv.NGX + 1) * (2 * sv.NGY + 1);
     const bool vh_lds = nvh <= BgjeShared<>::VHCAP;
     for (int c = t; c < Np; c += blockDim.x) ckey[c] = c < sv.N ? sv.nx[c] + (2 * sv.NGX + 1) * sv.ny[c] : 0;
+    bgje_roles(s);                               // ends with a barrier
     auto stage_vh = [&](int slice) {             // the caller synchronizes
         if (vh_lds)
             for (int e = t; e < nvh; e += blockDim.x) s.vh[e] = sv.vrec[(size_t)slice * nvh + e];
@@ -545,7 +552,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
             __syncthreads();
             build_block_keys(M, sv, jv, m - 1, m == Nz ? nullptr : hist + (size_t)m * NN, vsrc(m - 1), ckey);
             __syncthreads();
-            if (bgje_invert64(M, Np, s, so.cL)) grew = true;    // the solve's staging area is free here
+            if (bgje_invert64(M, Np, s, so.stage)) grew = true;    // the solves' staging area is free here
             for (int i = t; i < Np; i += blockDim.x) hsig[(size_t)(m - 1) * Np + i] = i;
         }
         __syncthreads();

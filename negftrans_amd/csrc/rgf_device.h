@@ -1056,11 +1056,57 @@ struct BgjeShared {
         };
         cd vh[VHCAP];            // outside the inversion: one slice's V-hat window (the kernel's)
     };
+    static constexpr int WLSZ = 2 * 16 * 64;        // one donor wave's panel rows W, in LDS
     int sigma[MAXNP];    // identity (the consumers read inverse[:, c] = Z[:, sigma[c]])
     int bad;
     int job;             // the workgroup's current job (kernels that take jobs in turn)
     double red[MAXW];
+    int simd[MAXW];      // the SIMD each wave runs on
+    int help[MAXW];      // trailing-update roles (bgje_roles): -1 none, -2 donor, >= 0 the donor helped
+    int wslot[MAXW];     // a donor's W area in the second staging buffer
 };
+
+// The trailing update's roles for balance across SIMDs (bgje_panel), from the SIMD each
+// wave runs on (HW_ID bits 5:4): on each SIMD with the most waves the last wave becomes a
+// donor, paired with the first wave of a SIMD with the fewest (at least one); every call
+// of bgje_invert64 in the workgroup then uses them.  Call once, by every thread; ends with
+// a barrier.  The result depends only on the wave placement, never on the data.
+template <class SH>
+__device__ __forceinline__ void bgje_roles(SH &s) {
+    const int t = threadIdx.x, w = t >> 6, nw = blockDim.x >> 6;
+    if ((t & 63) == 0) s.simd[w] = (__builtin_amdgcn_s_getreg((31 << 11) | (0 << 6) | 4) >> 4) & 3;
+    __syncthreads();
+    if (t == 0) {
+        int cnt[4] = {0, 0, 0, 0}, first[4] = {-1, -1, -1, -1}, last[4] = {-1, -1, -1, -1};
+        for (int v = 0; v < nw; ++v) {
+            const int c = s.simd[v];
+            ++cnt[c];
+            if (first[c] < 0) first[c] = v;
+            last[c] = v;
+            s.help[v] = -1;
+            s.wslot[v] = 0;
+        }
+        int mx = 0, mn = 1 << 30;
+        for (int c = 0; c < 4; ++c) {
+            mx = max(mx, cnt[c]);
+            mn = min(mn, cnt[c]);
+        }
+        if (mn >= 1 && mx > mn) {
+            int h = 0, l = 0, slot = 0;
+            while (slot < 2) {
+                while (h < 4 && cnt[h] != mx) ++h;
+                while (l < 4 && cnt[l] != mn) ++l;
+                if (h >= 4 || l >= 4) break;
+                s.help[last[h]] = -2;
+                s.wslot[last[h]] = slot++;
+                s.help[first[l]] = last[h];
+                ++h;
+                ++l;
+            }
+        }
+    }
+    __syncthreads();
+}
 
 // In-place FP64 Gauss-Jordan inverse (no pivoting) of the 16 x 16 block A[r * 17 + c] in
 // LDS by one wave: lane r + 16 h owns row r, columns [4 h, 4 h + 4).  Every lane reads its
@@ -1203,16 +1249,28 @@ __device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, co
             Pre[tt][g] = negf_f4{0.f, 0.f, 0.f, 0.f};
             Pim[tt][g] = negf_f4{0.f, 0.f, 0.f, 0.f};
         }
+    // panel loads and stores: one per-lane offset (from an opaque lane index, so that it is
+    // recomputed rather than kept live) plus wave-uniform scalar offsets, instead of 32
+    // per-lane addresses, which were spilled
+    auto panel_voff = [&]() {
+        int lx = threadIdx.x & 63;
+        asm volatile("" : "+v"(lx));
+        return voff(R0 + (lx & 15), k0 + (lx >> 4));
+    };
+    auto panel_soff = [&](int tt, int g, int r) { return (unsigned)(16 * tt + (16 * g + 4 * r) * Np) * 8u; };
+    {
+        const unsigned pv = panel_voff();
 #pragma unroll
-    for (int tt = 0; tt < 2; ++tt)
+        for (int tt = 0; tt < 2; ++tt)
 #pragma unroll
-        for (int g = 0; g < NSB; ++g)
+            for (int g = 0; g < NSB; ++g)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const cfl v = G::load(zr, voff(R0 + 16 * tt + j, k0 + 16 * g + 4 * r + k), 0u);
-                Pre[tt][g][r] = v.x;
-                Pim[tt][g][r] = v.y;
-            }
+                for (int r = 0; r < 4; ++r) {
+                    const cfl v = G::load(zr, pv, panel_soff(tt, g, r));
+                    Pre[tt][g][r] = v.x;
+                    Pim[tt][g][r] = v.y;
+                }
+    }
     // The trailing update's column chunks, skipping the panel's own.  The old pivot rows
     // Z_{K J} of a chunk go into LDS by DMA (buffer_load ... lds: one wave instruction
     // moves 64 floats, 32 pivot rows of one column, without passing through registers), so
@@ -1247,23 +1305,38 @@ __device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, co
         bgje_substep<1>(Pre, Pim, k0, NSB, w, s, bad, first);
     }
     // the panel columns take their final values
+    {
+        const unsigned pv = panel_voff();
 #pragma unroll
-    for (int tt = 0; tt < 2; ++tt)
+        for (int tt = 0; tt < 2; ++tt)
 #pragma unroll
-        for (int g = 0; g < NSB; ++g)
+            for (int g = 0; g < NSB; ++g)
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                G::store(cmk(Pre[tt][g][r], Pim[tt][g][r]), zr, voff(R0 + 16 * tt + j, k0 + 16 * g + 4 * r + k), 0u);
-    // trailing update, one chunk of CH columns at a time from the staged old pivot rows;
-    // rows in K start from zero
-    const bool zb = (R0 >= k0 && R0 < k0 + KW);
-    auto next_tile = [&](int ct) {
-        const int n = (ct + 16 == k0) ? k0 + 64 : ct + 16;
-        return n < Np ? n : ct;
-    };
+                for (int r = 0; r < 4; ++r) G::store(cmk(Pre[tt][g][r], Pim[tt][g][r]), zr, pv, panel_soff(tt, g, r));
+    }
+    // Trailing update, one chunk of CH columns at a time from the staged old pivot rows;
+    // rows in K start from zero.  Balance across SIMDs: the waves of the workgroup sit on
+    // the four SIMDs unevenly (ten waves: three, three, two, two), and the SIMDs with
+    // three waves set the pace of every chunk.  So a donor wave there (s.help[w] == -2)
+    // computes only the lower half of each chunk's column tiles, and a helper wave on a
+    // SIMD with fewer waves (s.help[w] = the donor) computes the upper half for the
+    // donor's rows as well, with the donor's panel rows W taken from LDS (the donor
+    // writes them after the panel store, ahead of the first chunk's barrier).
+    const int role = __builtin_amdgcn_readfirstlane(s.help[w]);
+    cfl *wlb = T2 + SH::TSZ + (role >= 0 ? s.wslot[role] : role == -2 ? s.wslot[w] : 0) * SH::WLSZ;
+    if (role == -2) {
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+            for (int g = 0; g < NSB; ++g)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) wlb[((16 * tt + 4 * g + r) << 6) + lane] = cmk(Pre[tt][g][r], Pim[tt][g][r]);
+    }
+    const int Rd = role >= 0 ? role << 5 : 0;
+    const bool zb = (R0 >= k0 && R0 < k0 + KW), zbd = (Rd >= k0 && Rd < k0 + KW);
     Acc nb0r, nb0i, nb1r, nb1i;
-    auto load_base = [&](int ct) {
-        const unsigned vo = voff(R0 + j, ct + 4 * k);
+    auto load_base = [&](int row0, int ct) {
+        const unsigned vo = voff(row0 + j, ct + 4 * k);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const cfl a0 = G::load(zr, vo, r * cstr), a1 = G::load(zr, vo + 128u, r * cstr);
@@ -1273,58 +1346,91 @@ __device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, co
             nb1i[r] = a1.y;
         }
     };
-    load_base(k0 == 0 ? 64 : 0);
+    load_base(R0, cfirst < Np ? cfirst : 0);
+    // One 32-row x 16-column tile at (row0, ct) of the chunk in tb: its base was prefetched
+    // into nb, the next item's base (nrow, ncol) is prefetched here, and with dma the wave's
+    // share of the next chunk's DMA goes out after that prefetch (the memory counter is in
+    // order: issued earlier, it would hold up the waits for the tile bases in between).
+    // LB: the panel rows W come from LDS (wl) instead of this wave's registers.
+    auto do_tile = [&](auto lds_b, const cfl *tb, cfl *tn, const int c0, const int cn, const int ct,
+                       const int row0, const bool zbt, const cfl *wl, const int nrow, const int ncol,
+                       const bool dma) {
+        constexpr bool LB = decltype(lds_b)::value;
+        // 3M complex products: with A = T (staged pivot rows) and B = W (panel),
+        //   c1 += Ar Br,  c2 += Ai Bi,  c3 += (Ar + Ai)(Br + Bi),
+        //   re = c1 - c2,  im = c3 - c1 - c2,
+        // from c1 = base re, c2 = 0, c3 = base re + base im: three matrix-core products per
+        // complex K-step instead of four
+        Acc c10, c20, c30, c11, c21, c31;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            c10[r] = zbt ? 0.f : nb0r[r];
+            c30[r] = zbt ? 0.f : nb0r[r] + nb0i[r];
+            c11[r] = zbt ? 0.f : nb1r[r];
+            c31[r] = zbt ? 0.f : nb1r[r] + nb1i[r];
+            c20[r] = 0.f;
+            c21[r] = 0.f;
+        }
+        load_base(nrow, ncol);
+        if (dma) stage(tn, cn);
+        __builtin_amdgcn_sched_barrier(0);          // keep the prefetch ahead of the MFMAs
+        int lx = threadIdx.x & 63;
+        asm volatile("" : "+v"(lx));                 // recompute, do not keep live
+        const cfl *tl = tb + (ct - c0 + (lx & 15)) * SH::LDT + (lx >> 4);
+#pragma unroll
+        for (int q = 0; q < 4 * NSB; ++q) {
+            const cfl tv = tl[4 * q];
+            const float ts = tv.x + tv.y;
+            float p0r, p0i, p1r, p1i;
+            if constexpr (LB) {
+                const cfl b0 = wl[(q << 6) + lx], b1 = wl[((16 + q) << 6) + lx];
+                p0r = b0.x;
+                p0i = b0.y;
+                p1r = b1.x;
+                p1i = b1.y;
+            } else {
+                p0r = Pre[0][q >> 2][q & 3];
+                p0i = Pim[0][q >> 2][q & 3];
+                p1r = Pre[1][q >> 2][q & 3];
+                p1i = Pim[1][q >> 2][q & 3];
+            }
+            c10 = G::mfma(tv.x, p0r, c10);
+            c11 = G::mfma(tv.x, p1r, c11);
+            c20 = G::mfma(tv.y, p0i, c20);
+            c21 = G::mfma(tv.y, p1i, c21);
+            c30 = G::mfma(ts, p0r + p0i, c30);
+            c31 = G::mfma(ts, p1r + p1i, c31);
+        }
+        const unsigned vo = voff(row0 + j, ct + 4 * k);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            G::store(cmk(c10[r] - c20[r], c30[r] - c10[r] - c20[r]), zr, vo, r * cstr);
+            G::store(cmk(c11[r] - c21[r], c31[r] - c11[r] - c21[r]), zr, vo + 128u, r * cstr);
+        }
+    };
     // One chunk from the buffer tb while the next one (from cn, if any) is staged into tn.
     // The barrier first: it waits for this wave's DMA of the chunk (a barrier waits for the
     // wave's memory operations) and for every wave to be done with tn (the chunk before).
-    // The next chunk's DMA is issued in the last tile, after that tile's base prefetch:
-    // the memory counter is in order, so a DMA issued earlier would hold up the waits for
-    // the tile bases in between.  The two buffers alternate in a loop unrolled by two, so
-    // that each LDS read names its buffer statically.
+    // The two buffers alternate in a loop unrolled by two, so that each LDS read names its
+    // buffer statically.
     auto run_chunk = [&](const cfl *tb, cfl *tn, const int c0, const int cn) {
         __builtin_amdgcn_s_waitcnt(0x0f70);             // vmcnt(0): this wave's DMA has landed
         __syncthreads();
         const int c1 = min(c0 + SH::CH, Np);
-        for (int ct = c0; ct < c1; ct += 16) {
-            // 3M complex products: with A = T (staged pivot rows) and B = W (panel),
-            //   c1 += Ar Br,  c2 += Ai Bi,  c3 += (Ar + Ai)(Br + Bi),
-            //   re = c1 - c2,  im = c3 - c1 - c2,
-            // from c1 = base re, c2 = 0, c3 = base re + base im: three matrix-core products per
-            // complex K-step instead of four
-            Acc c10, c20, c30, c11, c21, c31;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                c10[r] = zb ? 0.f : nb0r[r];
-                c30[r] = zb ? 0.f : nb0r[r] + nb0i[r];
-                c11[r] = zb ? 0.f : nb1r[r];
-                c31[r] = zb ? 0.f : nb1r[r] + nb1i[r];
-                c20[r] = 0.f;
-                c21[r] = 0.f;
-            }
-            load_base(next_tile(ct));
-            if (ct + 16 >= c1 && cn < Np) stage(tn, cn);
-            __builtin_amdgcn_sched_barrier(0);          // keep the prefetch ahead of the MFMAs
-            int lx = threadIdx.x & 63;
-            asm volatile("" : "+v"(lx));                 // recompute, do not keep live
-            const cfl *tl = tb + (ct - c0 + (lx & 15)) * SH::LDT + (lx >> 4);
-#pragma unroll
-            for (int q = 0; q < 4 * NSB; ++q) {
-                const cfl tv = tl[4 * q];
-                const float ts = tv.x + tv.y;
-                const float p0r = Pre[0][q >> 2][q & 3], p0i = Pim[0][q >> 2][q & 3];
-                const float p1r = Pre[1][q >> 2][q & 3], p1i = Pim[1][q >> 2][q & 3];
-                c10 = G::mfma(tv.x, p0r, c10);
-                c11 = G::mfma(tv.x, p1r, c11);
-                c20 = G::mfma(tv.y, p0i, c20);
-                c21 = G::mfma(tv.y, p1i, c21);
-                c30 = G::mfma(ts, p0r + p0i, c30);
-                c31 = G::mfma(ts, p1r + p1i, c31);
-            }
-            const unsigned vo = voff(R0 + j, ct + 4 * k);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                G::store(cmk(c10[r] - c20[r], c30[r] - c10[r] - c20[r]), zr, vo, r * cstr);
-                G::store(cmk(c11[r] - c21[r], c31[r] - c11[r] - c21[r]), zr, vo + 128u, r * cstr);
+        const int ch = c0 + 16 * ((c1 - c0) >> 5);      // the upper half of the tiles starts here
+        const int own_end = role == -2 ? ch : c1;
+        const int nxt = cn < Np ? cn : c0;               // first tile of the next chunk (in bounds)
+        for (int ct = c0; ct < own_end; ct += 16) {
+            const bool last = ct + 16 >= own_end;
+            const int nrow = (last && role >= 0) ? Rd : R0;
+            const int ncol = !last ? ct + 16 : role >= 0 ? ch : nxt;
+            do_tile(std::false_type{}, tb, tn, c0, cn, ct, R0, zb, wlb, nrow, ncol, last && cn < Np);
+        }
+        if (role >= 0) {
+            for (int ct = ch; ct < c1; ct += 16) {
+                const bool last = ct + 16 >= c1;
+                do_tile(std::true_type{}, tb, tn, c0, cn, ct, Rd, zbd, wlb, last ? R0 : Rd, last ? nxt : ct + 16,
+                        false);
             }
         }
     };
@@ -1342,7 +1448,8 @@ __device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, co
 
 // In-place inversion of Z (Np x Np, Np % 32 == 0, blockDim.x == 2 Np); on return
 // inverse = Z and s.sigma is the identity.  Returns 0, or 1 if the growth check failed.
-// T2: a second LDS staging buffer of SH::TSZ complex values, free for the call.
+// T2: a second LDS staging buffer of SH::TSZ + 2 SH::WLSZ complex values, free for the
+// call; s.help and s.wslot set by bgje_roles.
 template <class SH>
 __device__ __forceinline__ int bgje_invert64(cfl *__restrict__ Z, const int Np, SH &s, cfl *T2) {
     const int t = threadIdx.x, lane = t & 63;
