@@ -1318,10 +1318,12 @@ __device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, co
     // rows in K start from zero.  Balance across SIMDs: the waves of the workgroup sit on
     // the four SIMDs unevenly (ten waves: three, three, two, two), and the SIMDs with
     // three waves set the pace of every chunk.  So a donor wave there (s.help[w] == -2)
-    // computes only the lower half of each chunk's column tiles, and a helper wave on a
-    // SIMD with fewer waves (s.help[w] = the donor) computes the upper half for the
-    // donor's rows as well, with the donor's panel rows W taken from LDS (the donor
-    // writes them after the panel store, ahead of the first chunk's barrier).
+    // leaves the last column tile of each chunk to a helper wave on a SIMD with fewer
+    // waves (s.help[w] = the donor), which computes it for the donor's rows after its own
+    // tiles, with the donor's panel rows W taken from LDS (the donor writes them after the
+    // panel store, ahead of the first chunk's barrier).  One tile of four measured faster
+    // than two (9.50 s against 9.65 s per 2048-energy launch at Np = 320): the helper's
+    // tiles read W from LDS, and its SIMD then became the slower one.
     const int role = __builtin_amdgcn_readfirstlane(s.help[w]);
     cfl *wlb = T2 + SH::TSZ + (role >= 0 ? s.wslot[role] : role == -2 ? s.wslot[w] : 0) * SH::WLSZ;
     if (role == -2) {
@@ -1417,7 +1419,7 @@ __device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, co
         __builtin_amdgcn_s_waitcnt(0x0f70);             // vmcnt(0): this wave's DMA has landed
         __syncthreads();
         const int c1 = min(c0 + SH::CH, Np);
-        const int ch = c0 + 16 * ((c1 - c0) >> 5);      // the upper half of the tiles starts here
+        const int ch = c1 - 16;                          // the tile the helper takes
         const int own_end = role == -2 ? ch : c1;
         const int nxt = cn < Np ? cn : c0;               // first tile of the next chunk (in bounds)
         for (int ct = c0; ct < own_end; ct += 16) {
