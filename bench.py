@@ -6,11 +6,12 @@
 Workload: the disordered-wire transmission configuration N_pw = 317, N_z = 1000
 (Lx = Ly = 40 nm, dz = 0.25 nm, Gaussian-correlated disorder sigma = 3 meV, xi = 5 nm,
 seeded), energies from the uniform 0-30 meV grid of 4096 points.  A step is one
-batched launch of the transmission sweep over a strided quarter-or-less of the grid: B
-energies on the FP64 path (default 256, one job per resident workgroup slot) and B'
-on the mixed path (FP32 sweep + FP64 refinement; default 2048, eight jobs per slot, taken
-in turn: the launch ends when the last slot's last job does, and with eight jobs per
-slot that last round is a smaller share of the launch than with four).  Inputs (V-hat table, basis) are resident in HBM before the timed region.
+batched launch of the transmission sweep: B energies on the FP64 path (default 256, a
+strided sixteenth of the grid, one job per resident workgroup slot) and B' on the mixed
+path (FP32 sweep + FP64 refinement; default 4096, the whole grid in one call, sixteen jobs
+per slot taken in turn: the launch ends when the last slot's last job does, and with
+sixteen jobs per slot that last round is a smaller share of the launch than with eight
+or four).  Inputs (V-hat table, basis) are resident in HBM before the timed region.
 `value` is the faster path among those that pass the gate max |T - T_FP64| <= 1e-6
 (checked on the mixed path's last step), and "precisions" lists both.  Prints one
 JSON line.
@@ -31,9 +32,9 @@ PEAK_FP32_TFLOPS = 157.3     # MI355X dense FP32 matrix
 GATE = 1e-6                  # max |T_mixed - T_FP64| for the mixed path to count
 N_PW, N_Z = 317, 1000
 # HBM bytes per energy point of the dominant kernel: rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE over one launch (256 FP64 / 2048 mixed energies), scaled by a 1 GiB copy's counts (FETCH x2.0,
+# WRITE_SIZE over one launch (256 FP64 / 4096 mixed energies), scaled by a 1 GiB copy's counts (FETCH x2.0,
 # WRITE x1.0); the command lines and raw counts are in profiles/.
-TRAFFIC_BYTES_PER_POINT = {"fp64": 4.298e10, "mixed": 1.726e10}
+TRAFFIC_BYTES_PER_POINT = {"fp64": 4.298e10, "mixed": 1.665e10}
 # Matrix-core FLOPs executed per energy point.  FP64 kernel: the Gauss-Jordan inverse of
 # every slice (8 Np^3 at the padded size Np = 320) plus the open-row product on 16-row
 # tiles (8 * 16 * Np^2); SQ_INSTS_VALU_MFMA_MOPS_F64 reads 2.48e11 per point.  Mixed
@@ -114,8 +115,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0,
                     help="energies per FP64 step (default 256, one job per resident workgroup)")
     ap.add_argument("--batch-mixed", type=int, default=0,
-                    help="energies per mixed step (default 2048: workgroups take jobs in turn, eight "
-                         "per slot)")
+                    help="energies per mixed step (default 4096, the whole grid: workgroups take jobs in "
+                         "turn, sixteen per slot)")
     ap.add_argument("--config", choices=("disorder_c1", "qpc_c0"), default="disorder_c1",
                     help="qpc_c0: only the N = 61, N_z = 200 leg on the GPU (for profiling)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -155,7 +156,7 @@ def main():
     nx, ny = wl.basis()
     vrec = negf.local_potential_real_to_planewave(wl.V_real_eV / negf.HARTREE, NGX, NGY)
     Eall = wl.energies_hartree()
-    batches = {"fp64": args.batch or 256, "mixed": args.batch_mixed or 2048}
+    batches = {"fp64": args.batch or 256, "mixed": args.batch_mixed or 4096}
 
     def energies_for(step, batch):
         # a strided sweep of the 4096-point grid (spread over all of it for any batch), a
@@ -193,7 +194,7 @@ def main():
         assert (devm.N, devm.N_z) == (N_PW, N_Z)
         del devm
         if dev64 is not None:
-            # the gate: the last timed step's energies (a quarter of the grid) against the
+            # the gate: the last timed step's energies (the whole grid by default) against the
             # FP64 path at the same energies
             n = batches["mixed"]
             T64 = dev64.transmission(paths["mixed"]["E"][-n:])
