@@ -58,6 +58,22 @@ def test_mixed_small_vs_oracle(ng, N_z, seed, monkeypatch):
     assert np.all(np.abs(got - ref) <= 1e-6)
 
 
+@pytest.mark.parametrize("R2,seed", [(40.5, 1), (60.5, 2), (70.5, 4), (90.5, 6)])
+def test_mixed_wave_counts_vs_oracle(ng, R2, seed, monkeypatch):
+    """Block sizes whose waves sit unevenly on the four SIMDs: Np = 160, 192, 224 and 320
+    (5, 6, 7 and 10 waves), where the trailing update hands column tiles from one or two
+    donor waves to helper waves.  Against the CPU oracle at tolerance 1e-13."""
+    s, _ = small_system(N_z=6, seed=seed, R2=R2)
+    kin = O.kinetic(s.nx, s.ny, s.atomic.LX, s.atomic.LY, 0.0, 0.0)
+    E = np.linspace(0.05, 0.6, 5)
+    assert max(np.sum(2 * e - kin > 0) for e in E) <= 16
+    ref = O.transmission(s, E)
+    dev = mixed_device(ng, s, monkeypatch, tol=1e-13)
+    got = dev.transmission(E)
+    assert dev.last_status == 0
+    assert np.all(np.abs(got - ref) <= 1e-9 * (1.0 + np.abs(ref)))
+
+
 def test_mixed_qpc_c0_vs_fp64(ng, monkeypatch):
     """Config 0 geometry (N = 61, N_z = 200): mixed against the FP64 GPU path, within the
     1e-6 gate at the default tolerance and within 1e-9 at tolerance 1e-13."""
