@@ -258,6 +258,40 @@ def test_transmission_disorder_c1_one_sided(ng, monkeypatch):
         assert got[i] == pytest.approx(ref, rel=RTOL, abs=1e-12)
 
 
+def test_transmission_c3_full_depth(ng, monkeypatch):
+    """The largest configuration at its own size (N = 797, padded 800: the four-tile
+    matrix-core Gauss-Jordan; N_z = 2000), where the oracle would take minutes per
+    energy: two energies (5 and 9 open channels) through the two-sided sweep and, forced,
+    the one-sided sweep.  These are separate code paths (two workgroups meeting at the
+    middle slice, against one workgroup over all 2000 slices), so they must agree to
+    relative 1e-9, and 0 <= T <= n_open."""
+    from negftrans_amd import workloads
+    wl = workloads.disorder_c3()
+    NGX, NGY = wl.ngx()
+    vrec = ng.local_potential_real_to_planewave(wl.V_real_eV / O.HARTREE, NGX, NGY)
+    nx, ny = wl.basis()
+    a = wl.atomic()
+    E = wl.energies_hartree()[[600, 1023]]
+    kin = O.kinetic(np.asarray(nx), np.asarray(ny), a.LX, a.LY, 0.0, 0.0)
+    n_open = np.array([np.sum(2 * e - kin > 0) for e in E])
+    assert list(n_open) == [5, 9]
+    dev = ng.NegfDevice(nx, ny, a, vrec, NGX, NGY)
+    assert (dev.N, dev.N_z) == (797, 2000)
+    two = dev.transmission(E)
+    assert dev.last_status == 0
+    assert dev.last_variant == 3                  # two-sided, four-tile
+    dev.close()
+    monkeypatch.setenv("NEGF_DIAG_ONE_SIDED", "1")
+    dev = ng.NegfDevice(nx, ny, a, vrec, NGX, NGY)
+    monkeypatch.delenv("NEGF_DIAG_ONE_SIDED")
+    one = dev.transmission(E)
+    assert dev.last_status == 0
+    assert dev.last_variant == 2                  # one-sided, four-tile
+    dev.close()
+    np.testing.assert_allclose(two, one, rtol=1e-9, atol=1e-12)
+    assert np.all(two >= -1e-9) and np.all(two <= n_open + 1e-9)
+
+
 def test_density_c1_full_size(ng):
     """D mode at the metric configuration's size (N = 317, N_z = 1000): one contour node
     of the density (z in the upper half plane, weight 1) against the oracle's D-mode
