@@ -470,17 +470,23 @@ __device__ __forceinline__ void resid_scatter(const SysView &sv, const JobView &
     else resid_scatter_nb<16>(sv, jv, m, X, Y, ldb, nR, Rc, sg, vs, ckey, so);
 }
 
-// cL[sigma(i) 16 + b] = coef * V(i, b) (b < nR, else 0) for a row-major Np x ldb array V.
+// cL[i 16 + b] = coef * V(i, b) (b < nR, else 0) for a row-major Np x ldb array V (the
+// mixed sweep's column map is the identity).  blockDim.x == 2 Np, so every thread takes
+// exactly 8 entries; their loads are all issued before the first store.
 template <class C>
-__device__ __forceinline__ void scatter_rhs(cfl *cL, const C *V, const int *sg, int Np, int ldb, int nR, float coef) {
-    for (int e = threadIdx.x; e < Np * NEGF_NRH; e += blockDim.x) {
-        const int i = e >> 4, b = e & 15;
-        cfl v = cmk(0.0f, 0.0f);
-        if (b < nR) {
-            const cd x = to_cd(V[(size_t)i * ldb + b]);
-            v = cmk((float)x.x * coef, (float)x.y * coef);
-        }
-        cL[sg[i] * NEGF_NRH + b] = v;
+__device__ __forceinline__ void scatter_rhs(cfl *cL, const C *V, int Np, int ldb, int nR, float coef) {
+    constexpr int PER = NEGF_NRH / 2;
+    C x[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = threadIdx.x + u * blockDim.x, i = e >> 4, b = e & 15;
+        x[u] = b < nR ? V[(size_t)i * ldb + b] : C{};
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = threadIdx.x + u * blockDim.x, b = e & 15;
+        const cd xv = to_cd(x[u]);
+        cL[e] = b < nR ? cmk((float)xv.x * coef, (float)xv.y * coef) : cmk(0.0f, 0.0f);
     }
 }
 
@@ -571,7 +577,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
         }
         __syncthreads();
         for (int m = 2; m <= Nz; ++m) {
-            scatter_rhs(so.cL, X + (size_t)(m - 2) * Np * ldb, hsig + (size_t)(m - 1) * Np, Np, ldb, nR, -1.0f);
+            scatter_rhs(so.cL, X + (size_t)(m - 2) * Np * ldb, Np, ldb, nR, -1.0f);
             __syncthreads();
             negf_f4 ore[2], oim[2];
             gen_matvec(hist + (size_t)(m - 1) * NN, so.cL, Np, ore, oim);
@@ -614,7 +620,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
             for (int e = t; e < Np * ldb; e += blockDim.x) X[e] = cadd(X[e], to_cd(Y[e]));   // d_1 = y_1
             __syncthreads();
             for (int m = 2; m <= Nz; ++m) {           // d_m = y_m - g_m d_{m-1}, X_m += d_m
-                scatter_rhs(so.cL, Y + (size_t)(m - 2) * Np * ldb, hsig + (size_t)(m - 1) * Np, Np, ldb, nR, 1.0f);
+                scatter_rhs(so.cL, Y + (size_t)(m - 2) * Np * ldb, Np, ldb, nR, 1.0f);
                 __syncthreads();
                 negf_f4 ore[2], oim[2];
                 gen_matvec(hist + (size_t)(m - 1) * NN, so.cL, Np, ore, oim);
