@@ -304,6 +304,14 @@ struct MixedArgs {
     const int *order; // the order in which the jobs are taken (likely slow ones first)
 };
 
+// threadIdx.x through an empty asm: values derived from it are recomputed where they are
+// used instead of being hoisted out of the job loop and kept live across the sweep.
+__device__ __forceinline__ int opaque_tid() {
+    int x = threadIdx.x;
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 struct SolveShared {
     union {
         struct {
@@ -326,7 +334,7 @@ struct SolveShared {
 // im = p3 - p1 - p2 (three MFMAs per tile and K-step instead of four).
 __device__ __forceinline__ void gen_matvec(const cfl *Zg, const cfl *cL, int Np, negf_f4 (&ore)[2],
                                            negf_f4 (&oim)[2]) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int tid = opaque_tid(), lane = tid & 63, w = tid >> 6;
     const int i = lane & 15, k = lane >> 4;
     const __amdgpu_buffer_rsrc_t zr = negf_rsrc(const_cast<cfl *>(Zg), (unsigned)(Np * Np) * 8u);
     const unsigned vo = (unsigned)(((w << 5) + i) + k * Np) * 8u;
@@ -390,7 +398,7 @@ __device__ __forceinline__ void resid_scatter_nb(const SysView &sv, const JobVie
                                                  const cfl *Y, const int ldb, const int nR, const int *Rc,
                                                  const int *sg, const cd *vs, const int *ckey, SolveShared &so) {
     const int Np = sv.Np, N = sv.N, Nz = sv.Nz;
-    const int t = threadIdx.x, i = t >> 1, h = t & 1;
+    const int t = opaque_tid(), i = t >> 1, h = t & 1;
     const int slice = m - 1;
     const bool rin = i < N;
     const int rb = rin ? ckey[i] + sv.NGX + (2 * sv.NGX + 1) * sv.NGY : 0;
@@ -581,6 +589,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
             __syncthreads();
             negf_f4 ore[2], oim[2];
             gen_matvec(hist + (size_t)(m - 1) * NN, so.cL, Np, ore, oim);
+            const int tid_e = opaque_tid(), w = tid_e >> 6, jj = tid_e & 15, q = (tid_e >> 4) & 3;
             if (jj < nR) {
 #pragma unroll
                 for (int tt = 0; tt < 2; ++tt)
@@ -606,6 +615,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
                 __syncthreads();
                 negf_f4 ore[2], oim[2];
                 gen_matvec(hist + (size_t)(m - 1) * NN, so.cL, Np, ore, oim);
+                const int tid_e = opaque_tid(), w = tid_e >> 6, jj = tid_e & 15, q = (tid_e >> 4) & 3;
                 if (jj < nR) {
 #pragma unroll
                     for (int tt = 0; tt < 2; ++tt)
@@ -624,6 +634,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
                 __syncthreads();
                 negf_f4 ore[2], oim[2];
                 gen_matvec(hist + (size_t)(m - 1) * NN, so.cL, Np, ore, oim);
+                const int tid_e = opaque_tid(), w = tid_e >> 6, jj = tid_e & 15, q = (tid_e >> 4) & 3;
                 if (jj < nR) {
 #pragma unroll
                     for (int tt = 0; tt < 2; ++tt)
