@@ -304,14 +304,6 @@ struct MixedArgs {
     const int *order; // the order in which the jobs are taken (likely slow ones first)
 };
 
-// threadIdx.x through an empty asm: values derived from it are recomputed where they are
-// used instead of being hoisted out of the job loop and kept live across the sweep.
-__device__ __forceinline__ int opaque_tid() {
-    int x = threadIdx.x;
-    asm volatile("" : "+v"(x));
-    return x;
-}
-
 struct SolveShared {
     union {
         struct {

@@ -15,6 +15,15 @@
 typedef double2 cd;
 
 __device__ __forceinline__ cd cmk(double re, double im) { return make_double2(re, im); }
+// threadIdx.x through an empty asm: values derived from it are recomputed where they are
+// used instead of being hoisted out of an enclosing loop (a job or slice loop) and kept
+// live across it.
+__device__ __forceinline__ int opaque_tid() {
+    int x = threadIdx.x;
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 __device__ __forceinline__ cd cadd(cd a, cd b) { return cmk(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ cd csub(cd a, cd b) { return cmk(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ cd cmul(cd a, cd b) {
@@ -236,7 +245,7 @@ __device__ __forceinline__ void build_block_keys(cfl *__restrict__ M, const SysV
     constexpr int CBB = 8;
     const int nitems = Np * (Np / CBB);
 #pragma unroll 2
-    for (int it = threadIdx.x; it < nitems; it += blockDim.x) {
+    for (int it = opaque_tid(); it < nitems; it += blockDim.x) {
         const int r = it % Np, c0 = (it / Np) * CBB;
         const bool rin = r < N;
         const int rb = rin ? ckey[r] + base0 : 0;
@@ -1234,7 +1243,7 @@ __device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, co
     typedef GjePrec<float> G;
     typedef negf_f4 Acc;
     constexpr int KW = 16 * NSB;
-    const int t = threadIdx.x, lane = t & 63, nw = blockDim.x >> 6;
+    const int t = opaque_tid(), lane = t & 63, nw = blockDim.x >> 6;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int j = lane & 15, k = lane >> 4;
     const int R0 = w << 5;
