@@ -476,15 +476,16 @@ __device__ __forceinline__ void resid_scatter(const SysView &sv, const JobView &
 template <class C>
 __device__ __forceinline__ void scatter_rhs(cfl *cL, const C *V, int Np, int ldb, int nR, float coef) {
     constexpr int PER = NEGF_NRH / 2;
+    const int tid = opaque_tid();
     C x[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int e = threadIdx.x + u * blockDim.x, i = e >> 4, b = e & 15;
+        const int e = tid + u * blockDim.x, i = e >> 4, b = e & 15;
         x[u] = b < nR ? V[(size_t)i * ldb + b] : C{};
     }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int e = threadIdx.x + u * blockDim.x, b = e & 15;
+        const int e = tid + u * blockDim.x, b = e & 15;
         const cd xv = to_cd(x[u]);
         cL[e] = b < nR ? cmk((float)xv.x * coef, (float)xv.y * coef) : cmk(0.0f, 0.0f);
     }
@@ -501,7 +502,7 @@ __global__ __launch_bounds__(MAXT) void k_transmission_mixed(MixedArgs ma) {
     const SysView &sv = a.sv;
     const int Np = sv.Np, Nz = sv.Nz, ldb = ma.ldb;
     const size_t NN = (size_t)Np * Np;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int t = opaque_tid(), lane = t & 63, w = t >> 6;
     const int jj = lane & 15, q = lane >> 4;
     // basis keys for the V-hat window index of the block build and the residual, and the
     // slice's window staged in LDS when it fits (the Gauss-Jordan's staging area, free

@@ -1463,7 +1463,7 @@ __device__ __forceinline__ void bgje_panel(cfl *__restrict__ Z, const int Np, co
 // call; s.help and s.wslot set by bgje_roles.
 template <class SH>
 __device__ __forceinline__ int bgje_invert64(cfl *__restrict__ Z, const int Np, SH &s, cfl *T2) {
-    const int t = threadIdx.x, lane = t & 63;
+    const int t = opaque_tid(), lane = t & 63;
     for (int i = t; i < Np; i += blockDim.x) s.sigma[i] = i;
     if (t == 0) s.bad = 0;
     bool bad = false;
