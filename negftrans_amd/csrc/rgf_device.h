@@ -1123,7 +1123,7 @@ __device__ __forceinline__ void bgje_roles(SH &s) {
 // program order), so the steps need no barrier.  bad: a pivot reciprocal beyond
 // NEGF_BGJE_PMAX (or not finite).
 __device__ __forceinline__ void bgje_diag16(cd *A, bool &bad) {
-    const int lane = threadIdx.x & 63, r = lane & 15, h = lane >> 4;
+    const int lane = opaque_tid() & 63, r = lane & 15, h = lane >> 4;
     cd *row = A + r * 17 + 4 * h;
 #pragma unroll 1
     for (int J = 0; J < 16; ++J) {
