@@ -34,7 +34,7 @@ N_PW, N_Z = 317, 1000
 # HBM bytes per energy point of the dominant kernel: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE over one launch (256 FP64 / 4096 mixed energies), scaled by a 1 GiB copy's counts (FETCH x2.0,
 # WRITE x1.0); the command lines and raw counts are in profiles/.
-TRAFFIC_BYTES_PER_POINT = {"fp64": 4.298e10, "mixed": 1.620e10}
+TRAFFIC_BYTES_PER_POINT = {"fp64": 4.298e10, "mixed": 1.614e10}
 # Matrix-core FLOPs executed per energy point.  FP64 kernel: the Gauss-Jordan inverse of
 # every slice (8 Np^3 at the padded size Np = 320) plus the open-row product on 16-row
 # tiles (8 * 16 * Np^2); SQ_INSTS_VALU_MFMA_MOPS_F64 reads 2.48e11 per point.  Mixed
